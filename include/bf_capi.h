@@ -1,0 +1,292 @@
+/*
+ * bf_capi.h -- the C-ABI of libbeamform_hip.so, the MI355X (gfx950) build of
+ * the CHARTS voltage beamformer / dynamic tracker hot path.
+ *
+ * Plain C: pointers, sizes and an opaque handle type per stateful object.
+ * Device pointers are HIP device allocations; `stream` is a hipStream_t
+ * passed as void* (NULL = the default stream).  Every entry point returns a
+ * bf_status; on failure a thread-local message is available from
+ * bf_last_error_message().  The status codes map one-to-one onto the
+ * exception types the reference's C++ API throws:
+ *
+ *   BF_ERR_INVALID_ARGUMENT  std::invalid_argument (contract violations)
+ *   BF_ERR_OUT_OF_RANGE      std::out_of_range
+ *   BF_ERR_OVERFLOW          std::overflow_error (grid sizes)
+ *   BF_ERR_LOGIC             std::logic_error (workspace misuse)
+ *   BF_ERR_RUNTIME           std::runtime_error (HIP runtime failures)
+ *
+ * Validation always happens before anything is launched.  The C++ wrappers
+ * in include/beamformer/ rethrow these as the same exception types.
+ *
+ * Layouts (reference include/beamformer/indexing.hpp:13-66):
+ *   packed voltage [t][f][a] u8 (low nibble real, high nibble imag, int4)
+ *   weights        [b][f][a] complex f32 (re, im) -- "canonical"
+ *   tiled weights  [f][ceil(B/32)][a][32] complex f32, zero padded
+ *   intensity      [t][f][b] f32;  integrated [w][f][b] f32
+ *   int8 codes     [w][f][b];  params [wt][ft][bt] {offset, scale} f32
+ */
+#ifndef BF_CAPI_H
+#define BF_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status */
+typedef enum bf_status {
+    BF_OK = 0,
+    BF_ERR_INVALID_ARGUMENT = 1,
+    BF_ERR_OUT_OF_RANGE = 2,
+    BF_ERR_OVERFLOW = 3,
+    BF_ERR_LOGIC = 4,
+    BF_ERR_RUNTIME = 5
+} bf_status;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* bf_last_error_message(void);
+/* Library version string. */
+const char* bf_version(void);
+
+/* ------------------------------------------------------- dimensions */
+/* Reference: include/beamformer/config.hpp:74-79 (Dimensions). */
+typedef struct bf_dims {
+    size_t n_time;
+    size_t n_freq;
+    size_t n_ant;
+    size_t n_beams;
+} bf_dims;
+
+/* Reference: include/beamformer/config.hpp:81-94 (validate_dimensions):
+ * n_time > 0, n_freq == 336, n_ant in {32,64,128,256}, 1 <= n_beams <= 128. */
+bf_status bf_validate_dimensions(const bf_dims* dims);
+
+/* Reference: src/temporal_integration.cpp:16-26 (integrated_time_count). */
+bf_status bf_integrated_time_count(size_t n_time, size_t integration_spectra,
+                                   size_t* out);
+/* Reference: src/quantization.cpp:34-47 (quantization_parameter_count). */
+bf_status bf_quantization_parameter_count(const bf_dims* integrated_dims, size_t* out);
+
+/* ----------------------------------------------------- kernel choice */
+/* Reference: include/beamformer/cuda_stage_api.hpp:14-17
+ * (CudaBeamformerKernel {Direct, Tiled}).  DIRECT reads canonical weights,
+ * TILED the tiled layout; both run the plain FP32 fused kernel and keep the
+ * reference's integration rules (Direct: 320 only; Tiled: 10 or 320).
+ * The remaining selectors are additions of this build; all take canonical
+ * weights and accept any integration_spectra >= 1:
+ *   FP32       plain FP32 fused kernel (VALU FMA)
+ *   SPLIT_F16  FP16 MFMA, weights split hi+lo, FP32 accumulate (~FP32 accuracy)
+ *   F16        FP16 MFMA, single FP16 weights (~1e-3 relative)
+ *   AUTO       SPLIT_F16 where supported, else FP32 */
+typedef enum bf_kernel {
+    BF_KERNEL_DIRECT = 0,
+    BF_KERNEL_TILED = 1,
+    BF_KERNEL_FP32 = 2,
+    BF_KERNEL_SPLIT_F16 = 3,
+    BF_KERNEL_F16 = 4,
+    BF_KERNEL_AUTO = 5
+} bf_kernel;
+
+/* ---------------------------------------------------- stage launchers */
+/* Reference: include/beamformer/cuda_stage_api.hpp:19-24
+ * (launch_packed_beamformer).  Caller-owned device buffers; asynchronous on
+ * `stream`; allocates nothing and does not synchronize.  `weights` points at
+ * interleaved complex f32 in the layout the kernel selector names. */
+bf_status bf_launch_packed_beamformer(int kernel, void* stream,
+                                      const uint8_t* packed_voltage,
+                                      const float* weights, float* intensity,
+                                      const bf_dims* dims);
+
+/* Reference: include/beamformer/cuda_stage_api.hpp:26-34
+ * (launch_packed_integrated_beamformer): fused beamform + |.|^2 + temporal
+ * integration writing [W][F][B] directly. */
+bf_status bf_launch_packed_integrated_beamformer(int kernel, void* stream,
+                                                 const uint8_t* packed_voltage,
+                                                 const float* weights,
+                                                 float* integrated_intensity,
+                                                 const bf_dims* dims,
+                                                 size_t integration_spectra);
+
+/* Reference: include/beamformer/cuda_stage_api.hpp:36-42
+ * (launch_quantize_integrated_intensity): 1x16x16 chunk int8 codes; codes are
+ * bit-identical to the reference CPU quantiser applied to the same floats.
+ * `parameters` holds {offset, scale} pairs. */
+bf_status bf_launch_quantize_integrated_intensity(void* stream,
+                                                  const float* integrated_intensity,
+                                                  int8_t* quantized_intensity,
+                                                  float* parameters,
+                                                  const bf_dims* integrated_dims);
+
+/* Which kernel AUTO resolves to for these dimensions (a bf_kernel value). */
+bf_status bf_resolve_kernel(int kernel, const bf_dims* dims, size_t integration_spectra,
+                            int* resolved);
+
+/* ------------------------------------------------------- timings */
+/* Reference: include/beamformer/cuda_beamformer.hpp:22-31. */
+typedef struct bf_timings {
+    double setup_ms;
+    double host_to_device_ms;
+    double kernel_ms;
+    double temporal_integration_ms;
+    double quantization_ms;
+    double device_to_device_ms;
+    double device_to_host_ms;
+} bf_timings;
+
+/* ------------------------------------------------------- workspace */
+/* Reference: include/beamformer/cuda_beamformer.hpp:44-92
+ * (CudaBeamformerWorkspace).  Owns device buffers sized at creation, one
+ * stream and its events.  Methods are synchronous and report ms.  Request
+ * dims may shrink n_time and n_beams but not n_freq or n_ant.  Not
+ * thread-safe: one workspace per host thread. */
+typedef struct bf_workspace bf_workspace;
+
+typedef enum bf_output {
+    BF_OUTPUT_FLOAT32 = 0,
+    BF_OUTPUT_INT8 = 1
+} bf_output;
+
+/* integration_spectra == 0 means "no temporal integration". */
+bf_status bf_workspace_create(const bf_dims* capacity, int kernel,
+                              size_t integration_spectra, int output,
+                              bf_workspace** out);
+bf_status bf_workspace_destroy(bf_workspace* ws);
+bf_status bf_workspace_setup_ms(const bf_workspace* ws, double* out);
+bf_status bf_workspace_kernel(const bf_workspace* ws, int* out);
+bf_status bf_workspace_has_temporal_integration(const bf_workspace* ws, int* out);
+bf_status bf_workspace_has_int8_quantization(const bf_workspace* ws, int* out);
+bf_status bf_workspace_packed_voltage_capacity_bytes(const bf_workspace* ws, size_t* out);
+bf_status bf_workspace_upload_packed_voltage(bf_workspace* ws, const uint8_t* host,
+                                             size_t host_bytes, const bf_dims* dims,
+                                             double* ms);
+/* weight_count counts complex values in the kernel's layout. */
+bf_status bf_workspace_upload_weights(bf_workspace* ws, const float* host,
+                                      size_t weight_count, const bf_dims* dims,
+                                      double* ms);
+bf_status bf_workspace_run_kernel(bf_workspace* ws, const bf_dims* dims, double* ms);
+bf_status bf_workspace_download_intensity(bf_workspace* ws, float* host,
+                                          size_t host_count, const bf_dims* dims,
+                                          double* ms);
+bf_status bf_workspace_run_pipeline(bf_workspace* ws, const uint8_t* packed,
+                                    size_t packed_bytes, float* intensity,
+                                    size_t intensity_count, const bf_dims* dims,
+                                    bf_timings* timings);
+bf_status bf_workspace_run_integrated_pipeline(bf_workspace* ws, const uint8_t* packed,
+                                               size_t packed_bytes, float* integrated,
+                                               size_t integrated_count,
+                                               const bf_dims* dims, bf_timings* timings);
+bf_status bf_workspace_run_quantized_integrated_pipeline(
+    bf_workspace* ws, const uint8_t* packed, size_t packed_bytes, int8_t* codes,
+    size_t code_count, float* parameters, size_t parameter_count, const bf_dims* dims,
+    bf_timings* timings);
+
+/* ------------------------------------------------ device frame views */
+/* Reference: include/beamformer/cuda_frame.hpp:16-60 (CudaDataType,
+ * CudaBufferView, CudaFrameMetadata, CudaFrameView) and
+ * include/beamformer/config.hpp:29-41 (ShardDescriptor). */
+typedef enum bf_data_type {
+    BF_DTYPE_PACKED_INT4X2 = 0,
+    BF_DTYPE_COMPLEX_FLOAT32 = 1,
+    BF_DTYPE_FLOAT32 = 2,
+    BF_DTYPE_SIGNED_INT8 = 3
+} bf_data_type;
+
+typedef struct bf_shard {
+    size_t shard_id;
+    size_t shard_count;
+    size_t local_frequency_count;
+    size_t absolute_frequency_start;
+    uint64_t timestamp_start;
+    uint64_t timestamp_step;
+    uint64_t loss_mask_id;
+    int loss_mask_independent;
+} bf_shard;
+
+typedef struct bf_frame_view {
+    void* device_data;
+    size_t bytes;
+    int type; /* bf_data_type */
+    size_t rank;
+    size_t shape[4];
+    size_t strides_bytes[4];
+    uint64_t frame_id;
+    bf_shard shard;
+} bf_frame_view;
+
+/* Reference: src/cuda_beamformer.cu:930-1079 (run_device_frame): input and
+ * weights are caller device buffers, the final output is copied device to
+ * device into `output`; `quantization_parameters` is required for int8
+ * output and must be NULL otherwise. */
+bf_status bf_workspace_run_device_frame(bf_workspace* ws, const bf_frame_view* voltage,
+                                        const bf_frame_view* weights,
+                                        bf_frame_view* output,
+                                        bf_frame_view* quantization_parameters,
+                                        bf_timings* timings);
+
+/* --------------------------------------------------- one-shot API */
+/* Reference: include/beamformer/cuda_beamformer.hpp:94-123 and
+ * src/cuda_beamformer.cu:1081-1173: allocate-and-run on host buffers.
+ * `timings` may be NULL. */
+bf_status bf_beamform_packed_intensity(const uint8_t* packed, size_t packed_bytes,
+                                       const float* weights, size_t weight_count,
+                                       const bf_dims* dims, int kernel, float* out,
+                                       size_t out_count, bf_timings* timings);
+bf_status bf_beamform_packed_integrated_intensity(const uint8_t* packed,
+                                                  size_t packed_bytes,
+                                                  const float* weights,
+                                                  size_t weight_count,
+                                                  const bf_dims* dims,
+                                                  size_t integration_spectra, int kernel,
+                                                  float* out, size_t out_count,
+                                                  bf_timings* timings);
+bf_status bf_beamform_packed_quantized_integrated_intensity(
+    const uint8_t* packed, size_t packed_bytes, const float* weights, size_t weight_count,
+    const bf_dims* dims, size_t integration_spectra, int kernel, int8_t* codes,
+    size_t code_count, float* parameters, size_t parameter_count, bf_timings* timings);
+
+/* --------------------------------------------------- device plumbing */
+/* Minimal HIP plumbing so hosts without a HIP toolchain (ctypes, JNI, cgo)
+ * can own device buffers and streams.  Reference: the CUDA runtime calls
+ * the reference makes inside src/cuda_beamformer.cu:33-106. */
+bf_status bf_device_count(int* out);
+bf_status bf_device_info(char* name, size_t name_len, size_t* global_memory_bytes,
+                         int* compute_units, int* clock_khz);
+bf_status bf_device_malloc(size_t bytes, void** out);
+bf_status bf_device_free(void* ptr);
+bf_status bf_memcpy_htod(void* dst, const void* src, size_t bytes);
+bf_status bf_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+bf_status bf_memcpy_dtod(void* dst, const void* src, size_t bytes);
+bf_status bf_memset(void* dst, int value, size_t bytes);
+bf_status bf_stream_create(void** out);
+bf_status bf_stream_destroy(void* stream);
+bf_status bf_stream_synchronize(void* stream);
+bf_status bf_device_synchronize(void);
+/* Timing events on `stream`. */
+bf_status bf_event_create(void** out);
+bf_status bf_event_destroy(void* event);
+bf_status bf_event_record(void* event, void* stream);
+bf_status bf_event_elapsed_ms(void* start, void* end, float* ms);
+
+/* ------------------------------------------------ synthetic inputs */
+/* Seeded device generators for benchmark-size inputs (no reference
+ * counterpart; the reference generates on the host in
+ * src/synthetic_data.cpp).  Uniform: both nibbles uniform in [-8, 7] from a
+ * splitmix64 hash of (index, seed). */
+bf_status bf_generate_uniform_int4(void* stream, uint8_t* out, size_t count,
+                                   uint64_t seed);
+/* Point source plus complex Gaussian noise, rounded and clipped to int4:
+ * x = amplitude * exp(-j k p.d) + N(0, sigma^2) per component. */
+bf_status bf_generate_point_source_noise(void* stream, uint8_t* out, size_t n_time,
+                                         size_t n_freq, size_t n_ant,
+                                         const float* positions_device,
+                                         const float* frequencies_device,
+                                         const float* direction3, float amplitude,
+                                         float sigma, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BF_CAPI_H */

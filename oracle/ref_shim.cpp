@@ -1,0 +1,210 @@
+// ref_shim.cpp -- C entry points over the reference CPU core, compiled
+// together with the reference's own sources (see oracle/Makefile) into
+// oracle/_ref/libbfref.so.  TEST INFRASTRUCTURE ONLY: used to generate the
+// golden fixtures under tests/golden/ and, where present, as the timed CPU
+// baseline.  Every function catches the reference's exceptions and returns
+// -1 (invalid_argument), -2 (out_of_range) or -3 (other).
+#include "beamformer/beam_tracker.hpp"
+#include "beamformer/beam_tracker_opt_v2.hpp"
+#include "beamformer/cpu_beamformer.hpp"
+#include "beamformer/geometry.hpp"
+#include "beamformer/quantization.hpp"
+#include "beamformer/synthetic_data.hpp"
+#include "beamformer/temporal_integration.hpp"
+#include "beamformer/weights.hpp"
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+using namespace beamformer;
+
+namespace {
+
+template <typename Fn>
+int guarded(Fn&& fn) {
+    try {
+        fn();
+        return 0;
+    } catch (const std::invalid_argument&) {
+        return -1;
+    } catch (const std::out_of_range&) {
+        return -2;
+    } catch (...) {
+        return -3;
+    }
+}
+
+std::vector<Vec3> vec3s(const float* p, std::size_t n) {
+    std::vector<Vec3> v(n);
+    for (std::size_t i = 0; i < n; ++i) v[i] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+    return v;
+}
+
+void put_vec3s(const std::vector<Vec3>& v, float* out) {
+    for (std::size_t i = 0; i < v.size(); ++i) {
+        out[3 * i] = v[i][0];
+        out[3 * i + 1] = v[i][1];
+        out[3 * i + 2] = v[i][2];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_default_positions(std::size_t n_ant, float spacing, float* out) {
+    return guarded([&] { put_vec3s(default_positions(n_ant, spacing), out); });
+}
+
+int ref_channelized_frequencies(std::size_t n, float start, float width, float* out) {
+    return guarded([&] {
+        const auto f = channelized_frequencies(n, start, width);
+        std::memcpy(out, f.data(), n * sizeof(float));
+    });
+}
+
+int ref_direction_from_lm(float l, float m, float* out) {
+    return guarded([&] { put_vec3s({direction_from_lm(l, m)}, out); });
+}
+
+int ref_default_beam_grid(std::size_t n, float step, float m, float* out) {
+    return guarded([&] { put_vec3s(default_beam_grid(n, step, m), out); });
+}
+
+int ref_rectangular_beam_grid(std::size_t n_ant, float spacing, float design, float* out) {
+    return guarded([&] { put_vec3s(rectangular_beam_grid(n_ant, spacing, design), out); });
+}
+
+int ref_fft_beam_grid(std::size_t n_ant, std::size_t n_beams, float* out) {
+    return guarded([&] { put_vec3s(fft_beam_grid(n_ant, n_beams), out); });
+}
+
+int ref_generate_weights(std::size_t B, std::size_t F, std::size_t A, const float* pos,
+                         const float* freqs, const float* dirs, float* out) {
+    return guarded([&] {
+        const Dimensions d{1, F, A, B};
+        const auto w = generate_weights(d, vec3s(pos, A),
+                                        std::vector<float>(freqs, freqs + F), vec3s(dirs, B));
+        std::memcpy(out, w.data(), w.size() * sizeof(ComplexFloat));
+    });
+}
+
+int ref_generate_tiled_weights(std::size_t B, std::size_t F, std::size_t A,
+                               const float* pos, const float* freqs, const float* dirs,
+                               float* out) {
+    return guarded([&] {
+        const Dimensions d{1, F, A, B};
+        const auto w = generate_tiled_weights(
+            d, vec3s(pos, A), std::vector<float>(freqs, freqs + F), vec3s(dirs, B));
+        std::memcpy(out, w.data(), w.size() * sizeof(ComplexFloat));
+    });
+}
+
+int ref_beamform_packed(const std::uint8_t* x, const float* w, std::size_t T,
+                        std::size_t F, std::size_t A, std::size_t B, float* out) {
+    return guarded([&] {
+        const Dimensions d{T, F, A, B};
+        PackedVoltage p(x, x + T * F * A);
+        Weights wv(B * F * A);
+        std::memcpy(wv.data(), w, wv.size() * sizeof(ComplexFloat));
+        Intensities o(T * F * B);
+        cpu_beamform_packed_intensity_into(p, wv, d, o);
+        std::memcpy(out, o.data(), o.size() * sizeof(float));
+    });
+}
+
+int ref_integrate(const float* in, std::size_t T, std::size_t F, std::size_t A,
+                  std::size_t B, std::size_t S, float* out) {
+    return guarded([&] {
+        const Dimensions d{T, F, A, B};
+        Intensities i(in, in + T * F * B);
+        const auto o = cpu_integrate_intensity(i, d, S);
+        std::memcpy(out, o.data(), o.size() * sizeof(float));
+    });
+}
+
+int ref_quantize(const float* in, std::size_t Tw, std::size_t F, std::size_t A,
+                 std::size_t B, std::int8_t* codes, float* params) {
+    return guarded([&] {
+        const Dimensions d{Tw, F, A, B};
+        IntegratedIntensities i(in, in + Tw * F * B);
+        const auto q = cpu_quantize_integrated_intensity(i, d);
+        std::memcpy(codes, q.codes.data(), q.codes.size());
+        std::memcpy(params, q.parameters.data(),
+                    q.parameters.size() * sizeof(Int8QuantizationParameters));
+    });
+}
+
+int ref_make_noise(std::size_t T, std::size_t F, std::size_t A, std::uint32_t seed,
+                   std::uint8_t* out) {
+    return guarded([&] {
+        const auto v = make_noise(Dimensions{T, F, A, 1}, seed);
+        std::memcpy(out, v.data(), v.size());
+    });
+}
+
+int ref_make_point_source(std::size_t T, std::size_t F, std::size_t A, const float* pos,
+                          const float* freqs, const float* dir, float amp,
+                          std::uint8_t* out) {
+    return guarded([&] {
+        const auto v = make_point_source(Dimensions{T, F, A, 1}, vec3s(pos, A),
+                                         std::vector<float>(freqs, freqs + F),
+                                         Vec3{dir[0], dir[1], dir[2]}, amp);
+        std::memcpy(out, v.data(), v.size());
+    });
+}
+
+int ref_make_moving_point_source(std::size_t T, std::size_t F, std::size_t A,
+                                 const float* pos, const float* freqs, float l0, float m0,
+                                 float rl, float rm, float amp, std::uint8_t* out) {
+    return guarded([&] {
+        TrackerTrajectoryConfig tr{direction_from_lm(l0, m0), {rl, rm}};
+        const auto v = beam_tracker_make_moving_point_source(
+            Dimensions{T, F, A, 1}, vec3s(pos, A), std::vector<float>(freqs, freqs + F),
+            tr, amp);
+        std::memcpy(out, v.data(), v.size());
+    });
+}
+
+int ref_beam_tracker(const std::uint8_t* x, std::size_t T, std::size_t F, std::size_t A,
+                     float l0, float m0, float rl, float rm, std::size_t S, float* out) {
+    return guarded([&] {
+        TrackerConfig tc;
+        tc.trajectory.direction_start = direction_from_lm(l0, m0);
+        tc.trajectory.direction_rate_per_sample = {rl, rm};
+        tc.integration_spectra = S;
+        const Dimensions d{T, F, A, 1};
+        PackedVoltage p(x, x + T * F * A);
+        Intensities o(T * F);
+        beam_tracker_cpu_packed_intensity_into(p, d, tc, o);
+        std::memcpy(out, o.data(), o.size() * sizeof(float));
+    });
+}
+
+int ref_beam_tracker_opt_v2(const std::uint8_t* x, std::size_t T, std::size_t F,
+                            std::size_t A, float l0, float m0, float rl, float rm,
+                            std::size_t S, float* out) {
+    return guarded([&] {
+        TrackerConfig tc;
+        tc.trajectory.direction_start = direction_from_lm(l0, m0);
+        tc.trajectory.direction_rate_per_sample = {rl, rm};
+        tc.integration_spectra = S;
+        const Dimensions d{T, F, A, 1};
+        PackedVoltage p(x, x + T * F * A);
+        Intensities o(T * F);
+        beam_tracker_opt_v2_cpu_packed_intensity_into(p, d, tc, o);
+        std::memcpy(out, o.data(), o.size() * sizeof(float));
+    });
+}
+
+int ref_tracker_direction(float l0, float m0, float rl, float rm, std::size_t t,
+                          float* out) {
+    return guarded([&] {
+        TrackerTrajectoryConfig tr{direction_from_lm(l0, m0), {rl, rm}};
+        put_vec3s({tracker_direction(tr, t)}, out);
+    });
+}
+
+}  // extern "C"

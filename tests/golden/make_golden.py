@@ -1,0 +1,131 @@
+"""Writes the golden fixtures under tests/golden/ from the reference CPU core.
+
+The reference's C++ CPU sources are compiled in place by oracle/Makefile
+(`make -C oracle ref`) into oracle/_ref/libbfref.so; this script calls that
+library through oracle/ref_shim.cpp and stores small input/output vectors.
+Each case mirrors a case of the reference's own tests:
+
+  weights_a32_b5        tests/cpu/test_cpu_beamformer.cpp:76-110
+  noise_bytes           src/synthetic_data.cpp:107-119 (make_noise, seed 4242)
+  beamform_noise_a32_b5 tests/cpu/test_cpu_beamformer.cpp:207-237 (seeded noise)
+  beamform_noise_a64_b7 tests/cuda/test_cuda_packed.cu:198-202 (fft grid, seed 9876)
+  point_source_a32      tests/cpu/test_cpu_beamformer.cpp:162-181
+  integrate_ramp        tests/cpu/test_temporal_integration.cpp:59-90
+  quantize_ramp         tests/cpu/test_quantization.cpp:38-62
+  tracker_moving_a32    tests/cuda/test_cuda_tracker_v2.cpp:45-58 (shortened)
+
+Run:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle as O  # noqa: E402
+
+F = 336
+
+
+def main() -> None:
+    R = O.ref_lib()
+    if R is None:
+        raise SystemExit("oracle/_ref/libbfref.so missing: run `make -C oracle ref`")
+
+    def pos(A, spacing=0.6):
+        p = np.zeros((A, 3), np.float32)
+        assert R.ref_default_positions(A, spacing, p) == 0
+        return p
+
+    freqs = np.zeros(F, np.float32)
+    assert R.ref_channelized_frequencies(F, 300e6, 300e3, freqs) == 0
+
+    def beam_grid(n):
+        d = np.zeros((n, 3), np.float32)
+        assert R.ref_default_beam_grid(n, 0.02, 0.0, d) == 0
+        return d
+
+    def weights(A, dirs):
+        B = len(dirs)
+        w = np.zeros((B, F, A, 2), np.float32)
+        assert R.ref_generate_weights(B, F, A, pos(A), freqs, dirs, w) == 0
+        return w
+
+    def noise(T, A, seed):
+        x = np.zeros(T * F * A, np.uint8)
+        assert R.ref_make_noise(T, F, A, seed, x) == 0
+        return x.reshape(T, F, A)
+
+    def beamform(x, w):
+        T, _, A = x.shape
+        B = w.shape[0]
+        out = np.zeros((T, F, B), np.float32)
+        assert R.ref_beamform_packed(x.reshape(-1), w.reshape(-1), T, F, A, B, out) == 0
+        return out
+
+    save = lambda name, **kw: np.savez_compressed(os.path.join(HERE, name + ".npz"), **kw)
+
+    # weights + tiled weights on the default 5-beam grid (A = 32)
+    dirs5 = beam_grid(5)
+    w5 = weights(32, dirs5)
+    tiled = np.zeros((F, 1, 32, 32, 2), np.float32)
+    assert R.ref_generate_tiled_weights(5, F, 32, pos(32), freqs, dirs5, tiled) == 0
+    save("weights_a32_b5", positions=pos(32), frequencies=freqs, directions=dirs5,
+         weights=w5, tiled=tiled)
+
+    x = noise(2, 32, 4242)
+    save("noise_bytes", seed=np.uint32(4242), shape=np.array(x.shape), bytes=x)
+    save("beamform_noise_a32_b5", x=x, directions=dirs5, intensity=beamform(x, w5))
+
+    dirs7 = np.zeros((7, 3), np.float32)
+    assert R.ref_fft_beam_grid(64, 7, dirs7) == 0
+    x64 = noise(10, 64, 9876)
+    save("beamform_noise_a64_b7", x=x64, directions=dirs7,
+         intensity=beamform(x64, weights(64, dirs7)))
+
+    d004 = np.zeros(3, np.float32)
+    assert R.ref_direction_from_lm(0.04, 0.0, d004) == 0
+    ps = np.zeros(F * 32, np.uint8)
+    assert R.ref_make_point_source(1, F, 32, pos(32), freqs, d004, 4.0, ps) == 0
+    ps = ps.reshape(1, F, 32)
+    save("point_source_a32", direction=d004, x=ps, intensity=beamform(ps, w5))
+
+    T, B, S = 7, 3, 3
+    t, f, b = np.meshgrid(np.arange(T), np.arange(F), np.arange(B), indexing="ij")
+    ramp = (100 * t + 10 * f + b).astype(np.float32)
+    integ = np.zeros((3, F, B), np.float32)
+    assert R.ref_integrate(ramp.reshape(-1), T, F, 32, B, S, integ) == 0
+    save("integrate_ramp", input=ramp, spectra=np.int64(S), output=integ)
+
+    T, B = 2, 17
+    t, f, b = np.meshgrid(np.arange(T), np.arange(F), np.arange(B), indexing="ij")
+    qin = (1000 * t + 10 * f + b).astype(np.float32)
+    codes = np.zeros((T, F, B), np.int8)
+    params = np.zeros((T * 21 * 2, 2), np.float32)
+    assert R.ref_quantize(qin.reshape(-1), T, F, 32, B, codes, params) == 0
+    save("quantize_ramp", input=qin, codes=codes, params=params)
+
+    T, A, S = 40, 32, 16
+    l0, m0, rl, rm = 0.02, -0.01, 1.0e-4, 5.0e-5
+    xm = np.zeros(T * F * A, np.uint8)
+    assert R.ref_make_moving_point_source(T, F, A, pos(A), freqs, l0, m0, rl, rm, 4.0, xm) == 0
+    out = np.zeros((T, F, 1), np.float32)
+    assert R.ref_beam_tracker(xm, T, F, A, l0, m0, rl, rm, S, out) == 0
+    out2 = np.zeros_like(out)
+    assert R.ref_beam_tracker_opt_v2(xm, T, F, A, l0, m0, rl, rm, S, out2) == 0
+    assert np.array_equal(out, out2)
+    save("tracker_moving_a32", x=xm.reshape(T, F, A),
+         trajectory=np.array([l0, m0, rl, rm], np.float32), spectra=np.int64(S),
+         intensity=out)
+    for name in sorted(os.listdir(HERE)):
+        if name.endswith(".npz"):
+            print(f"{name:32s} {os.path.getsize(os.path.join(HERE, name)):>9d} bytes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,92 @@
+"""CPU-side checks of the C-ABI boundary: the library loads, exports every
+symbol include/*.h declares, and rejects contract violations with the
+reference's error classes before touching the GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from beamform_project_amd import _capi, api
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(bf_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_symbols()
+    assert len(names) > 40
+    lib = C.CDLL(_capi.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_header():
+    assert declared_symbols() <= set(_capi.SIGNATURES)
+
+
+def test_dimension_contract():
+    # include/beamformer/config.hpp:81-94
+    api.validate_dimensions(api.Dimensions(1, 336, 32, 1))
+    for bad in (api.Dimensions(0, 336, 64, 1), api.Dimensions(1, 672, 64, 1),
+                api.Dimensions(1, 336, 48, 1), api.Dimensions(1, 336, 64, 0),
+                api.Dimensions(1, 336, 64, 129)):
+        with pytest.raises(_capi.InvalidArgument):
+            api.validate_dimensions(bad)
+
+
+def test_integration_rules_of_reference_selectors():
+    d = api.Dimensions(640, 336, 64, 4)
+    # src/cuda_beamformer.cu:418-433: Tiled 10/320, Direct 320 only.
+    with pytest.raises(_capi.InvalidArgument, match="Direct"):
+        api.launch_packed_integrated_beamformer(api.Kernel.DIRECT, None, 256, 256, 256, d, 10)
+    with pytest.raises(_capi.InvalidArgument, match="10 or 320"):
+        api.launch_packed_integrated_beamformer(api.Kernel.TILED, None, 256, 256, 256, d, 20)
+    with pytest.raises(_capi.InvalidArgument):
+        api.launch_packed_integrated_beamformer(api.Kernel.SPLIT_F16, None, 256, 256, 256, d, 7)
+    with pytest.raises(_capi.InvalidArgument):
+        api.launch_packed_integrated_beamformer(api.Kernel.FP32, None, 256, 256, 256, d, 0)
+    with pytest.raises(_capi.InvalidArgument, match="null"):
+        api.launch_packed_beamformer(api.Kernel.FP32, None, None, 256, 256, d)
+    with pytest.raises(_capi.InvalidArgument, match="aligned"):
+        api.launch_packed_beamformer(api.Kernel.FP32, None, 257, 256, 256, d)
+    with pytest.raises(_capi.InvalidArgument, match="selector"):
+        api.launch_packed_beamformer(99, None, 256, 256, 256, d)
+
+
+def test_counts_and_kernel_resolution():
+    assert api.integrated_time_count(32768, 10) == 3277
+    assert api.integrated_time_count(65536, 320) == 205
+    with pytest.raises(_capi.InvalidArgument):
+        api.integrated_time_count(10, 0)
+    assert api.quantization_parameter_count(api.Dimensions(2, 336, 32, 17)) == 84
+    d = api.Dimensions(32768, 336, 64, 128)
+    assert api.resolve_kernel(api.Kernel.AUTO, d, 10) == api.Kernel.SPLIT_F16
+    assert api.resolve_kernel(api.Kernel.AUTO, d, 7) == api.Kernel.FP32
+    assert api.resolve_kernel(api.Kernel.TILED, d, 10) == api.Kernel.TILED
+
+
+def test_workspace_rejects_int8_without_integration():
+    with pytest.raises(_capi.InvalidArgument, match="int8"):
+        api.BeamformerWorkspace(api.Dimensions(10, 336, 64, 4), api.Kernel.TILED, None,
+                                api.Output.INT8)
+
+
+def test_error_message_is_thread_local_and_cleared():
+    with pytest.raises(_capi.InvalidArgument):
+        api.validate_dimensions(api.Dimensions(0, 336, 64, 1))
+    assert b"n_time" in _capi.LIB.bf_last_error_message()
+    api.validate_dimensions(api.Dimensions(1, 336, 64, 1))
+    assert _capi.LIB.bf_last_error_message() == b""
