@@ -36,6 +36,13 @@ COMMON_FLAGS = [
 ]
 
 
+# Translation units whose float arithmetic must stay separately rounded
+# (bit-exact parity with the reference CPU code): no a*b+c contraction.
+PER_FILE_FLAGS = {
+    "k_quant.hip": ["-ffp-contract=off"],
+}
+
+
 def _sources():
     return sorted(
         f for f in os.listdir(CSRC) if f.endswith(".hip") or f.endswith(".cpp")
@@ -51,11 +58,12 @@ def _headers():
 def _compile(src: str, force: bool) -> str:
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, src + ".o")
-    newest = max([os.path.getmtime(path)] + [os.path.getmtime(h) for h in _headers()])
+    newest = max([os.path.getmtime(path), os.path.getmtime(os.path.abspath(__file__))]
+                 + [os.path.getmtime(h) for h in _headers()])
     if not force and os.path.exists(obj) and os.path.getmtime(obj) >= newest:
         return obj
     lang = ["-x", "hip"]
-    cmd = [HIPCC, *lang, *COMMON_FLAGS, "-c", path, "-o", obj]
+    cmd = [HIPCC, *lang, *COMMON_FLAGS, *PER_FILE_FLAGS.get(src, []), "-c", path, "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)

@@ -10,20 +10,24 @@
 // and runs the K loop twice over the same A operand (products are exact in
 // the FP32 accumulator), recovering ~22 significant bits per weight.
 //
-// Instruction: v_mfma_f32_16x16x32_f16.  A wave owns one frequency and
-// NT N-tiles (8 beams each, re/im interleaved); its weights live in VGPRs
-// for the whole time range it processes.  Each lane loads A/4 contiguous
-// packed bytes of one time row per 16-row tile and converts them to FP16
-// pairs with a byte permute, an xor and one packed subtraction.
+// Instruction: v_mfma_f32_16x16x32_f16.  A workgroup of 4 waves owns one
+// frequency, one chunk of time and 4*NT*8 beams; each wave keeps the
+// weights of NT N-tiles (8 beams each, re/im interleaved) in VGPRs for the
+// whole chunk.  The packed int4 rows of a stage (ST tiles of 16 spectra)
+// are loaded once per workgroup with coalesced dword loads, converted to
+// FP16 pairs (byte permute, xor, one packed subtraction) and written to LDS
+// in the MFMA A-operand layout, so each lane reads its operand with one
+// 16-byte LDS load per k-step.  The next stage's loads are issued before the
+// current stage is computed.
 //
 // Output layout of a 16x16 MFMA: lane l holds column l%16 for rows
 // 4*(l/16)+r, r = 0..3.  Tile rows are mapped to time so that row group
-// G = l/16 of the super-tile covers GROUP_ROWS consecutive spectra: each
-// lane then accumulates whole integration windows in registers.
-//   MODE 0 (no integration): super-tile = 1 tile, row 4G+r -> t = base+4G+r
-//   MODE 1 (S == 10): 5 tiles, G covers 20 spectra = 2 windows
-//   MODE 2 (S % 4 == 0): S/4 tiles, G covers one window of S spectra
-// Re and im of a beam sit in adjacent lanes; one quad-permute DPP adds the
+// G = l/16 of a super-tile covers consecutive spectra: each lane then
+// accumulates whole integration windows in registers.
+//   MODE 0 (no integration): tile i, row 4G+r -> t = 16i + 4G + r
+//   MODE 1 (S == 10): super-tile of 5 tiles, G covers 20 spectra (2 windows)
+//   MODE 2 (S % 4 == 0): super-tile of S/4 tiles, G covers one window
+// Re and im of a beam sit in adjacent lanes; a quad-permute DPP adds the
 // two squared sums.
 #include "bf_kernels.hpp"
 
@@ -31,6 +35,8 @@ namespace bfhip {
 namespace {
 
 constexpr int kWaveSize = 64;
+constexpr int kWaves = 4;
+constexpr int kThreads = kWaves * kWaveSize;
 constexpr int kXcds = 8;
 
 __device__ __forceinline__ float dpp_swap_pairs(float v) {
@@ -39,68 +45,14 @@ __device__ __forceinline__ float dpp_swap_pairs(float v) {
         float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true));
 }
 
-template <int A>
-struct RowLoad;
-template <>
-struct RowLoad<32> {  // 8 bytes per lane
-    static __device__ __forceinline__ void load(const uint8_t* p, bool ok, unsigned d[2]) {
-        uint2 v = ok ? *reinterpret_cast<const uint2*>(p) : make_uint2(0u, 0u);
-        d[0] = v.x;
-        d[1] = v.y;
-    }
-};
-template <>
-struct RowLoad<64> {  // 16 bytes per lane
-    static __device__ __forceinline__ void load(const uint8_t* p, bool ok, unsigned d[4]) {
-        uint4 v = ok ? *reinterpret_cast<const uint4*>(p) : make_uint4(0u, 0u, 0u, 0u);
-        d[0] = v.x;
-        d[1] = v.y;
-        d[2] = v.z;
-        d[3] = v.w;
-    }
-};
-template <>
-struct RowLoad<128> {  // 32 bytes per lane
-    static __device__ __forceinline__ void load(const uint8_t* p, bool ok, unsigned d[8]) {
-        const uint4* q = reinterpret_cast<const uint4*>(p);
-        uint4 v0 = ok ? q[0] : make_uint4(0u, 0u, 0u, 0u);
-        uint4 v1 = ok ? q[1] : make_uint4(0u, 0u, 0u, 0u);
-        d[0] = v0.x; d[1] = v0.y; d[2] = v0.z; d[3] = v0.w;
-        d[4] = v1.x; d[5] = v1.y; d[6] = v1.z; d[7] = v1.w;
-    }
-};
-template <>
-struct RowLoad<256> {  // 64 bytes per lane
-    static __device__ __forceinline__ void load(const uint8_t* p, bool ok, unsigned d[16]) {
-        const uint4* q = reinterpret_cast<const uint4*>(p);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            uint4 v = ok ? q[i] : make_uint4(0u, 0u, 0u, 0u);
-            d[4 * i + 0] = v.x;
-            d[4 * i + 1] = v.y;
-            d[4 * i + 2] = v.z;
-            d[4 * i + 3] = v.w;
-        }
-    }
-};
-
+// Time of row (G, r) of global tile i.
 template <int MODE>
-struct TileGeom;
-template <>
-struct TileGeom<0> {
-    static constexpr int kTiles = 1;
-    static __device__ __forceinline__ int group_rows(int) { return 4; }
-};
-template <>
-struct TileGeom<1> {
-    static constexpr int kTiles = 5;
-    static __device__ __forceinline__ int group_rows(int) { return 20; }
-};
-template <>
-struct TileGeom<2> {
-    static constexpr int kTiles = 0;  // S / 4, at run time
-    static __device__ __forceinline__ int group_rows(int S) { return S; }
-};
+__device__ __forceinline__ int tile_time(int i, int G, int r, int S) {
+    if (MODE == 0) return 16 * i + 4 * G + r;
+    if (MODE == 1) return 80 * (i / 5) + 20 * G + 4 * (i % 5) + r;
+    const int tps = S >> 2;
+    return 4 * S * (i / tps) + S * G + 4 * (i % tps) + r;
+}
 
 // Loads the B operand (weights) of one wave into registers.
 template <int A, int NT, bool SPLIT>
@@ -141,61 +93,114 @@ __device__ __forceinline__ void load_weights(const float2* __restrict__ w, int F
     }
 }
 
-template <int A, int NT, bool SPLIT, int MODE>
-__global__ __launch_bounds__(256) void mfma_fused_kernel(
+template <int A, int NT, bool SPLIT, int MODE, int ST>
+__global__ __launch_bounds__(kThreads) void mfma_fused_kernel(
     const uint8_t* __restrict__ x, const float2* __restrict__ w, float* __restrict__ out,
-    int T, int F, int B, int S, int W, int n_supertiles, int st_per_chunk, int n_items,
+    int T, int F, int B, int S, int W, int tiles_total, int tiles_per_chunk, int n_items,
     int per_xcd) {
-    constexpr int KS = A / 16;         // k-steps of 32 (16 antennas x re/im)
-    constexpr int BYTES = A / 4;       // packed bytes per lane per row
-    const int lane = threadIdx.x & (kWaveSize - 1);
-    const int wave = threadIdx.x / kWaveSize;
-    // XCD-aware order: consecutive work items (adjacent frequencies of one
-    // time chunk, which share 128-byte lines) run on the same XCD / L2.
+    constexpr int KS = A / 16;                     // k-steps of 32 (16 antennas x re/im)
+    constexpr int TILE_BYTES = KS * kWaveSize * 16; // LDS bytes per 16-row tile
+    constexpr int DW_ROW = A / 4;                   // packed dwords per row
+    constexpr int DW_TILE = 16 * DW_ROW;
+    constexpr int DW_STAGE = ST * DW_TILE;
+    constexpr int DW_THREAD = (DW_STAGE + kThreads - 1) / kThreads;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[ST * TILE_BYTES];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWaveSize - 1);
+    const int wave = tid / kWaveSize;
+    // XCD-aware order: workgroups are dispatched round-robin over the 8
+    // XCDs, so blockIdx.x % 8 names the XCD.  Each XCD takes a contiguous
+    // range of frequency-major items: all time chunks of a channel run on
+    // one XCD back to back (its 64 KB of weights stay in that L2) and the
+    // neighbouring channel, which shares the 128-byte lines of every packed
+    // row, follows on the same XCD.
     const int item = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
-    if (item >= n_items) return;
-    const int f = item % F;
-    const int chunk = item / F;
-    const int beam0 = wave * NT * 8;
-    if (beam0 >= B) return;
+    if (item >= n_items) return;  // whole workgroup
+    const int n_chunks = n_items / F;
+    const int f = item / n_chunks;
+    const int chunk = item - f * n_chunks;
+    const int beam0 = (blockIdx.y * kWaves + wave) * NT * 8;
+    const bool has_beams = beam0 < B;
 
     bf_half8 bhi[NT][KS];
     bf_half8 blo[NT][KS];
-    load_weights<A, NT, SPLIT>(w, F, B, f, beam0, lane, bhi, blo);
+    if (has_beams) load_weights<A, NT, SPLIT>(w, F, B, f, beam0, lane, bhi, blo);
 
-    const int rho = lane & 15;  // A row / D column
-    const int g = lane >> 4;    // k group of the A operand / row group of D
-    const int grows = TileGeom<MODE>::group_rows(S);
-    const int st_rows = 4 * grows;
-    const int tiles = MODE == 2 ? S / 4 : TileGeom<MODE>::kTiles;
-    const int a_row_off = (rho >> 2) * grows + (rho & 3);
-    const uint8_t* xf = x + (size_t)f * A + g * BYTES;
+    const int tile_begin = chunk * tiles_per_chunk;
+    int tile_end = tile_begin + tiles_per_chunk;
+    tile_end = tile_end < tiles_total ? tile_end : tiles_total;
     const size_t row_stride = (size_t)F * A;
+    const uint8_t* xf = x + (size_t)f * A;
 
-    const int st_begin = chunk * st_per_chunk;
-    int st_end = st_begin + st_per_chunk;
-    st_end = st_end < n_supertiles ? st_end : n_supertiles;
-
-    for (int st = st_begin; st < st_end; ++st) {
-        const int base = st * st_rows;
-        float win[NT][2];
+    // Stage loader: dword q of the stage -> (tile, row, dword in row).
+    unsigned pre[DW_THREAD];
+    auto load_stage = [&](int stage_tile0) {
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            win[nt][0] = 0.0f;
-            win[nt][1] = 0.0f;
+        for (int k = 0; k < DW_THREAD; ++k) {
+            const int q = tid + k * kThreads;
+            unsigned v = 0u;
+            if (q < DW_STAGE) {
+                const int tl = q / DW_TILE;
+                const int rem = q - tl * DW_TILE;
+                const int rho = rem / DW_ROW;
+                const int j = rem - rho * DW_ROW;
+                const int t = tile_time<MODE>(stage_tile0 + tl, rho >> 2, rho & 3, S);
+                if (t < T)
+                    v = *reinterpret_cast<const unsigned*>(xf + (size_t)t * row_stride + 4 * j);
+            }
+            pre[k] = v;
         }
-#pragma unroll(MODE == 1 ? 5 : 1)
-        for (int i = 0; i < tiles; ++i) {
-            const int t = base + a_row_off + 4 * i;
-            unsigned d[BYTES / 4];
-            RowLoad<A>::load(xf + (size_t)t * row_stride, t < T, d);
-            bf_half8 av[KS];
+    };
+    auto store_stage = [&]() {
 #pragma unroll
-            for (int s = 0; s < KS; ++s) {
+        for (int k = 0; k < DW_THREAD; ++k) {
+            const int q = tid + k * kThreads;
+            if (q < DW_STAGE) {
+                const int tl = q / DW_TILE;
+                const int rem = q - tl * DW_TILE;
+                const int rho = rem / DW_ROW;
+                const int j = rem - rho * DW_ROW;
+                const int g = j / (A / 16);
+                const int s = j - g * (A / 16);
                 unsigned h2[4];
-                bf_int4x4_to_half2x4(d[s], h2);
+                bf_int4x4_to_half2x4(pre[k], h2);
                 bf_uint4v u = {h2[0], h2[1], h2[2], h2[3]};
-                av[s] = __builtin_bit_cast(bf_half8, u);
+                *reinterpret_cast<bf_uint4v*>(lds + tl * TILE_BYTES +
+                                              (s * kWaveSize + rho + 16 * g) * 16) = u;
+            }
+        }
+    };
+
+    float win[NT][2];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) win[nt][0] = win[nt][1] = 0.0f;
+    const int col = lane & 15;
+    const int G = lane >> 4;
+    const int tps = MODE == 2 ? (S >> 2) : 5;
+
+    if (tile_begin < tile_end) load_stage(tile_begin);
+    for (int st0 = tile_begin; st0 < tile_end; st0 += ST) {
+        __syncthreads();  // previous stage fully consumed
+        store_stage();
+        __syncthreads();  // stage visible to all waves
+        if (st0 + ST < tile_end) load_stage(st0 + ST);  // in flight during compute
+        if (!has_beams) continue;
+        // One tile ahead: the A operand of tile tl+1 is read from LDS while
+        // the MFMAs of tile tl run.
+        bf_half8 av[KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+            av[s] = *reinterpret_cast<const bf_half8*>(lds + (s * kWaveSize + lane) * 16);
+#pragma unroll 1
+        for (int tl = 0; tl < ST; ++tl) {
+            const int i = st0 + tl;
+            bf_half8 an[KS];
+            if (tl + 1 < ST) {
+#pragma unroll
+                for (int s = 0; s < KS; ++s)
+                    an[s] = *reinterpret_cast<const bf_half8*>(
+                        lds + (tl + 1) * TILE_BYTES + (s * kWaveSize + lane) * 16);
             }
             bf_float4 acc[NT];
 #pragma unroll
@@ -213,94 +218,161 @@ __global__ __launch_bounds__(256) void mfma_fused_kernel(
                                                                          acc[nt], 0, 0, 0);
                 }
             }
+#pragma unroll
+            for (int s = 0; s < KS; ++s) av[s] = an[s];
             if (MODE == 0) {
-                // One output per row: t = base + 4*g + r.
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
-                    const int b = beam0 + nt * 8 + (rho >> 1);
+                    const int b = beam0 + nt * 8 + (col >> 1);
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const float sq = acc[nt][r] * acc[nt][r];
                         const float p = sq + dpp_swap_pairs(sq);
-                        const int tt = base + 4 * g + r;
-                        if ((rho & 1) == 0 && b < B && tt < T)
-                            out[((size_t)tt * F + f) * B + b] = p;
+                        const int t = tile_time<0>(i, G, r, S);
+                        if ((col & 1) == 0 && b < B && t < T)
+                            out[((size_t)t * F + f) * B + b] = p;
                     }
                 }
-            } else {
+                continue;
+            }
+            const int ii = MODE == 1 ? tl : (i % tps);  // tile within super-tile
+            if (MODE == 1 && tl == 2) {
+                // spectra 8, 9 close window 0; 10, 11 open window 1
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
+                    win[nt][0] = fmaf(acc[nt][0], acc[nt][0], win[nt][0]);
+                    win[nt][0] = fmaf(acc[nt][1], acc[nt][1], win[nt][0]);
+                    win[nt][1] = fmaf(acc[nt][2], acc[nt][2], win[nt][1]);
+                    win[nt][1] = fmaf(acc[nt][3], acc[nt][3], win[nt][1]);
+                }
+            } else {
+                const int q = (MODE == 1 && tl > 2) ? 1 : 0;  // wave-uniform
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const float v = acc[nt][r];
-                        // MODE 1: local spectrum 4i+r of the group's 20 -> window (4i+r)/10.
-                        if (MODE == 1 && 4 * i + r >= 10)
-                            win[nt][1] = fmaf(v, v, win[nt][1]);
-                        else
-                            win[nt][0] = fmaf(v, v, win[nt][0]);
-                    }
+                for (int nt = 0; nt < NT; ++nt) {
+                    float a = 0.0f;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) a = fmaf(acc[nt][r], acc[nt][r], a);
+                    if (q) win[nt][1] += a;
+                    else win[nt][0] += a;
                 }
             }
-        }
-        if (MODE != 0) {
-            constexpr int kWinPerGroup = MODE == 1 ? 2 : 1;
+            if (ii == tps - 1) {
+                constexpr int kWinPerGroup = MODE == 1 ? 2 : 1;
+                const int super = i / tps;
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                const int b = beam0 + nt * 8 + (rho >> 1);
+                for (int nt = 0; nt < NT; ++nt) {
+                    const int b = beam0 + nt * 8 + (col >> 1);
 #pragma unroll
-                for (int q = 0; q < kWinPerGroup; ++q) {
-                    const float p = win[nt][q] + dpp_swap_pairs(win[nt][q]);
-                    const int wi = base / S + g * kWinPerGroup + q;
-                    if ((rho & 1) == 0 && b < B && wi < W)
-                        out[((size_t)wi * F + f) * B + b] = p;
+                    for (int q = 0; q < kWinPerGroup; ++q) {
+                        const float p = win[nt][q] + dpp_swap_pairs(win[nt][q]);
+                        const int wi = super * 4 * kWinPerGroup + G * kWinPerGroup + q;
+                        if ((col & 1) == 0 && b < B && wi < W)
+                            out[((size_t)wi * F + f) * B + b] = p;
+                        win[nt][q] = 0.0f;
+                    }
                 }
             }
         }
     }
 }
 
+struct Plan {
+    int mode, st, tiles_total, tiles_per_chunk, n_items, per_xcd, beam_blocks, W;
+};
+
+Plan make_plan(int T, int F, int B, int S, int NT) {
+    Plan p{};
+    p.mode = S == 0 ? 0 : (S == 10 ? 1 : 2);
+    int tiles_per_super = 1;
+    if (p.mode == 0) {
+        p.st = 4;
+        p.W = T;
+    } else if (p.mode == 1) {
+        p.st = 5;
+        tiles_per_super = 5;
+        p.W = (int)ceil_div((size_t)T, 10);
+    } else {
+        tiles_per_super = S / 4;
+        p.st = tiles_per_super % 4 == 0 ? 4 : 1;
+        p.W = (int)ceil_div((size_t)T, (size_t)S);
+    }
+    const int rows_super = 16 * tiles_per_super;
+    const int n_super = (int)ceil_div((size_t)T, (size_t)rows_super);
+    p.tiles_total = n_super * tiles_per_super;
+    if (p.mode == 0) p.tiles_total = (int)ceil_div((size_t)p.tiles_total, 4) * 4;
+    // Chunk length: a whole number of super-tiles and stages, chosen to
+    // minimise (rounds of resident workgroups) x (tiles per workgroup +
+    // a fixed start-up cost for loading the weights), with 512 workgroups
+    // resident (256 CUs x 2).
+    int unit = tiles_per_super;
+    while (unit % p.st) unit += tiles_per_super;
+    const long resident = 512, startup_tiles = 12;
+    long best_cost = -1;
+    int per_chunk = unit;
+    for (int c = unit; c <= 640 && c <= p.tiles_total + unit; c += unit) {
+        const long chunks = (long)ceil_div((size_t)p.tiles_total, (size_t)c);
+        const long rounds = (long)ceil_div((size_t)(chunks * F), (size_t)resident);
+        const long cost = rounds * (c + startup_tiles);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            per_chunk = c;
+        }
+    }
+    p.tiles_per_chunk = per_chunk;
+    const int n_chunks = (int)ceil_div((size_t)p.tiles_total, (size_t)per_chunk);
+    p.n_items = n_chunks * F;
+    p.per_xcd = (int)ceil_div((size_t)p.n_items, kXcds);
+    p.beam_blocks = (int)ceil_div((size_t)B, (size_t)(kWaves * NT * 8));
+    return p;
+}
+
 template <int A, int NT, bool SPLIT>
-void launch_mode(hipStream_t stream, const uint8_t* x, const float* w, float* out, int T,
-                 int F, int B, int S, int mode) {
-    const int st_rows = mode == 0 ? 16 : (mode == 1 ? 80 : 4 * S);
-    const int W = mode == 0 ? T : (int)ceil_div((size_t)T, (size_t)S);
-    const int n_st = (int)ceil_div((size_t)T, (size_t)st_rows);
-    // ~1280 spectra per wave keeps the weight load amortised and gives
-    // several waves per SIMD at the benchmark shapes.
-    int st_per_chunk = (int)ceil_div(1280, (size_t)st_rows);
-    if (st_per_chunk < 1) st_per_chunk = 1;
-    const int n_chunks = (int)ceil_div((size_t)n_st, (size_t)st_per_chunk);
-    const int n_items = n_chunks * F;
-    const int per_xcd = (int)ceil_div((size_t)n_items, kXcds);
-    const int waves = (int)ceil_div((size_t)B, (size_t)(NT * 8));
-    const dim3 grid((unsigned)(per_xcd * kXcds));
-    const dim3 block((unsigned)(waves * kWaveSize));
+void launch_nt(hipStream_t stream, const uint8_t* x, const float* w, float* out, int T, int F,
+               int B, int S) {
+    const Plan p = make_plan(T, F, B, S, NT);
+    const dim3 grid((unsigned)(p.per_xcd * kXcds), (unsigned)p.beam_blocks);
+    const dim3 block(kThreads);
     const float2* w2 = reinterpret_cast<const float2*>(w);
-    switch (mode) {
-    case 0:
-        hipLaunchKernelGGL((mfma_fused_kernel<A, NT, SPLIT, 0>), grid, block, 0, stream, x, w2,
-                           out, T, F, B, S, W, n_st, st_per_chunk, n_items, per_xcd);
-        break;
-    case 1:
-        hipLaunchKernelGGL((mfma_fused_kernel<A, NT, SPLIT, 1>), grid, block, 0, stream, x, w2,
-                           out, T, F, B, S, W, n_st, st_per_chunk, n_items, per_xcd);
-        break;
-    default:
-        hipLaunchKernelGGL((mfma_fused_kernel<A, NT, SPLIT, 2>), grid, block, 0, stream, x, w2,
-                           out, T, F, B, S, W, n_st, st_per_chunk, n_items, per_xcd);
-        break;
+#define BF_LAUNCH(MODE_, ST_)                                                                 \
+    hipLaunchKernelGGL((mfma_fused_kernel<A, NT, SPLIT, MODE_, ST_>), grid, block, 0, stream, \
+                       x, w2, out, T, F, B, S, p.W, p.tiles_total, p.tiles_per_chunk,          \
+                       p.n_items, p.per_xcd)
+    if (p.mode == 0) {
+        BF_LAUNCH(0, 4);
+    } else if (p.mode == 1) {
+        BF_LAUNCH(1, 5);
+    } else if (p.st == 4) {
+        BF_LAUNCH(2, 4);
+    } else {
+        BF_LAUNCH(2, 1);
+    }
+#undef BF_LAUNCH
+}
+
+// Beams per wave: as many 8-beam N-tiles as the beam count needs, capped so
+// that the split weights take at most 128 VGPRs per lane.
+template <int A, bool SPLIT>
+void launch_a(hipStream_t stream, const uint8_t* x, const float* w, float* out, int T, int F,
+              int B, int S) {
+    constexpr int kMaxNT = A == 32 ? 4 : (A == 64 ? 4 : (A == 128 ? 2 : 1));
+    const int need = (int)ceil_div((size_t)B, (size_t)(kWaves * 8));
+    if (need <= 1 || kMaxNT == 1) {
+        launch_nt<A, 1, SPLIT>(stream, x, w, out, T, F, B, S);
+    } else if (need <= 2 || kMaxNT == 2) {
+        launch_nt<A, (kMaxNT >= 2 ? 2 : 1), SPLIT>(stream, x, w, out, T, F, B, S);
+    } else {
+        launch_nt<A, kMaxNT, SPLIT>(stream, x, w, out, T, F, B, S);
     }
 }
 
 template <bool SPLIT>
 void launch_split(hipStream_t stream, const uint8_t* x, const float* w, float* out, int T,
-                  int F, int A, int B, int S, int mode) {
-    // Beams per wave chosen so the weights take 128 VGPRs (split) per lane.
+                  int F, int A, int B, int S) {
     switch (A) {
-    case 32: launch_mode<32, 8, SPLIT>(stream, x, w, out, T, F, B, S, mode); break;
-    case 64: launch_mode<64, 4, SPLIT>(stream, x, w, out, T, F, B, S, mode); break;
-    case 128: launch_mode<128, 2, SPLIT>(stream, x, w, out, T, F, B, S, mode); break;
-    case 256: launch_mode<256, 1, SPLIT>(stream, x, w, out, T, F, B, S, mode); break;
+    case 32: launch_a<32, SPLIT>(stream, x, w, out, T, F, B, S); break;
+    case 64: launch_a<64, SPLIT>(stream, x, w, out, T, F, B, S); break;
+    case 128: launch_a<128, SPLIT>(stream, x, w, out, T, F, B, S); break;
+    case 256: launch_a<256, SPLIT>(stream, x, w, out, T, F, B, S); break;
     default: invalid("n_ant must be 32, 64, 128, or 256");
     }
 }
@@ -318,13 +390,12 @@ void launch_mfma_fused(hipStream_t stream, bool split, const uint8_t* x, const f
                        float* out, size_t T, size_t F, size_t A, size_t B, size_t S) {
     if (!mfma_supported(A, B, S))
         invalid("the MFMA beamformer supports integration of 10 spectra or a multiple of 4");
-    if (T > (size_t)INT32_MAX / 2 || T * F > ((size_t)1 << 40))
+    if (T > (size_t)INT32_MAX / 8 || S > (size_t)INT32_MAX / 8 || T * F > ((size_t)1 << 40))
         fail(BF_ERR_OVERFLOW, "MFMA beamformer grid exceeds the supported size");
-    const int mode = S == 0 ? 0 : (S == 10 ? 1 : 2);
     if (split)
-        launch_split<true>(stream, x, w, out, (int)T, (int)F, (int)A, (int)B, (int)S, mode);
+        launch_split<true>(stream, x, w, out, (int)T, (int)F, (int)A, (int)B, (int)S);
     else
-        launch_split<false>(stream, x, w, out, (int)T, (int)F, (int)A, (int)B, (int)S, mode);
+        launch_split<false>(stream, x, w, out, (int)T, (int)F, (int)A, (int)B, (int)S);
     hip_check(hipGetLastError(), "mfma_fused_kernel launch");
 }
 

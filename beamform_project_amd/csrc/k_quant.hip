@@ -8,9 +8,6 @@
 // 127); non-finite -> -128; scale == 0 -> 0.  Every operation is written
 // as a separately rounded IEEE operation (no contraction, correctly rounded
 // division) so the codes are bit-identical to the CPU quantiser's.
-//
-// One wave per chunk: lane l covers frequency 16*ft + l/4 and beams
-// 16*bt + 4*(l%4) .. +3.
 #include "bf_kernels.hpp"
 
 #include <cmath>
@@ -29,71 +26,86 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// One workgroup per (window t, 16-channel tile fi): the 16 x B block of
+// floats is contiguous in [t][f][b], so it is read and the codes written
+// with fully coalesced accesses through LDS; wave w then quantises chunks
+// w, w+4, ... of the block (lane l: channel l/4, beams 4*(l%4) .. +3).
 __global__ __launch_bounds__(256) void quantize_kernel(const float* __restrict__ in,
                                                        int8_t* __restrict__ codes,
-                                                       float2* __restrict__ params, int Tw,
-                                                       int F, int B, int ft, int bt,
-                                                       long n_chunks) {
+                                                       float2* __restrict__ params, int F,
+                                                       int B, int ft, int bt) {
 #pragma clang fp contract(off)
-    const long chunk = (long)blockIdx.x * 4 + threadIdx.x / 64;
-    if (chunk >= n_chunks) return;  // wave-uniform
+    __shared__ float tile[16 * 128];
+    __shared__ int8_t ctile[16 * 128];
+    const long blk = blockIdx.x;
+    const int fi = (int)(blk % ft);
+    const long t = blk / ft;
+    const int f0 = fi * 16;
+    const int nf = F - f0 < 16 ? F - f0 : 16;
+    const size_t base = ((size_t)t * F + f0) * B;
+    const int n = nf * B;
+    for (int e = threadIdx.x; e < n; e += 256) tile[e] = in[base + e];
+    __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int bi = (int)(chunk % bt);
-    const long tf = chunk / bt;
-    const int fi = (int)(tf % ft);
-    const long t = tf / ft;
-    const int f = fi * 16 + lane / 4;
-    const int bb = bi * 16 + (lane % 4) * 4;
-    float v[4];
-    bool ok[4];
-    float mn = INFINITY, mx = -INFINITY;
-    bool nonfinite = false;
+    const int wave = threadIdx.x >> 6;
+    const int lf = lane / 4;
+    for (int bi = wave; bi < bt; bi += 4) {
+        const int bb = bi * 16 + (lane % 4) * 4;
+        float v[4];
+        bool ok[4];
+        float mn = INFINITY, mx = -INFINITY;
+        bool nonfinite = false;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        ok[k] = f < F && bb + k < B;
-        v[k] = ok[k] ? in[((size_t)t * F + f) * B + bb + k] : 0.0f;
-        if (ok[k]) {
-            if (!isfinite(v[k])) {
-                nonfinite = true;
-            } else {
-                mn = fminf(mn, v[k]);
-                mx = fmaxf(mx, v[k]);
+        for (int k = 0; k < 4; ++k) {
+            ok[k] = lf < nf && bb + k < B;
+            v[k] = ok[k] ? tile[lf * B + bb + k] : 0.0f;
+            if (ok[k]) {
+                if (!isfinite(v[k])) {
+                    nonfinite = true;
+                } else {
+                    mn = fminf(mn, v[k]);
+                    mx = fmaxf(mx, v[k]);
+                }
             }
         }
-    }
-    nonfinite = __any(nonfinite);
-    mn = wave_min(mn);
-    mx = wave_max(mx);
-    constexpr float out_min = -127.0f - 0.5f;
-    constexpr float out_max = 127.0f + 0.5f;
-    constexpr float out_range = out_max - out_min;
-    float offset = 0.0f, scale = 0.0f;
-    if (!nonfinite && isfinite(mn) && isfinite(mx)) {
-        const float span = __fsub_rn(mx, mn);
-        scale = __fdiv_rn(span, out_range);
-        const float shift = __fmul_rn(out_min, scale);
-        offset = __fsub_rn(mn, shift);
-        if (!isfinite(offset) || !isfinite(scale)) {
-            offset = 0.0f;
-            scale = 0.0f;
+        nonfinite = __any(nonfinite);
+        mn = wave_min(mn);
+        mx = wave_max(mx);
+        constexpr float out_min = -127.0f - 0.5f;
+        constexpr float out_max = 127.0f + 0.5f;
+        constexpr float out_range = out_max - out_min;
+        float offset = 0.0f, scale = 0.0f;
+        if (!nonfinite && isfinite(mn) && isfinite(mx)) {
+            // This translation unit is compiled without contraction.
+            const float span = mx - mn;
+            scale = span / out_range;
+            const float shift = out_min * scale;
+            offset = mn - shift;
+            if (!isfinite(offset) || !isfinite(scale)) {
+                offset = 0.0f;
+                scale = 0.0f;
+            }
         }
-    }
-    if (lane == 0) params[chunk] = make_float2(offset, scale);
+        if (lane == 0) params[(t * ft + fi) * bt + bi] = make_float2(offset, scale);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (!ok[k]) continue;
-        int code = 0;
-        if (!isfinite(v[k])) {
-            code = -128;
-        } else if (scale != 0.0f) {
-            const float q = __fdiv_rn(__fsub_rn(v[k], offset), scale);
-            float r = __builtin_rintf(q);
-            r = r < -127.0f ? -127.0f : r;
-            r = r > 127.0f ? 127.0f : r;
-            code = (int)r;
+        for (int k = 0; k < 4; ++k) {
+            if (!ok[k]) continue;
+            int code = 0;
+            if (!isfinite(v[k])) {
+                code = -128;
+            } else if (scale != 0.0f) {
+                const float d = v[k] - offset;
+                const float q = d / scale;
+                float r = __builtin_rintf(q);
+                r = r < -127.0f ? -127.0f : r;
+                r = r > 127.0f ? 127.0f : r;
+                code = (int)r;
+            }
+            ctile[lf * B + bb + k] = (int8_t)code;
         }
-        codes[((size_t)t * F + f) * B + bb + k] = (int8_t)code;
     }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += 256) codes[base + e] = ctile[e];
 }
 
 __device__ __forceinline__ uint64_t splitmix(uint64_t key, uint64_t seed) {
@@ -152,12 +164,13 @@ __global__ void point_source_noise_kernel(uint8_t* __restrict__ out, size_t T, i
 void launch_quantize(hipStream_t stream, const float* in, int8_t* codes, float* params,
                      size_t Tw, size_t F, size_t B) {
     const size_t ft = ceil_div(F, 16), bt = ceil_div(B, 16);
-    const size_t n_chunks = Tw * ft * bt;
-    if (ceil_div(n_chunks, 4) > (size_t)UINT32_MAX)
+    const size_t n_blocks = Tw * ft;
+    if (B > kMaximumBeams) invalid("n_beams must be between 1 and 128");
+    if (n_blocks > (size_t)UINT32_MAX)
         fail(BF_ERR_OVERFLOW, "quantization grid exceeds the supported size");
-    hipLaunchKernelGGL(quantize_kernel, dim3((unsigned)ceil_div(n_chunks, 4)), dim3(256), 0,
-                       stream, in, codes, reinterpret_cast<float2*>(params), (int)Tw, (int)F,
-                       (int)B, (int)ft, (int)bt, (long)n_chunks);
+    hipLaunchKernelGGL(quantize_kernel, dim3((unsigned)n_blocks), dim3(256), 0, stream, in,
+                       codes, reinterpret_cast<float2*>(params), (int)F, (int)B, (int)ft,
+                       (int)bt);
     hip_check(hipGetLastError(), "quantize_kernel launch");
 }
 

@@ -333,5 +333,3 @@ def test_full_shard_sampled_frequencies():
     ref = O.integrate_f64(i64, T, len(sample), B, S)
     err = rel_err_floored(got[:, sample], ref, incoherent_floor(x[:, sample], A, S))
     assert err <= SPLIT_TOL, err
-    # the last window holds 8 spectra
-    assert np.all(got[-1] < got[:-1].mean(axis=0) * 2)
