@@ -40,6 +40,7 @@ COMMON_FLAGS = [
 # (bit-exact parity with the reference CPU code): no a*b+c contraction.
 PER_FILE_FLAGS = {
     "k_quant.hip": ["-ffp-contract=off"],
+    "k_tracker.hip": ["-ffp-contract=off"],
 }
 
 

@@ -246,6 +246,96 @@ bf_status bf_beamform_packed_quantized_integrated_intensity(
     const bf_dims* dims, size_t integration_spectra, int kernel, int8_t* codes,
     size_t code_count, float* parameters, size_t parameter_count, bf_timings* timings);
 
+/* ------------------------------------------------------- dynamic tracker */
+/* Reference: include/beamformer/beam_tracker.hpp:33-57 (TrackerConfig).
+ * The trajectory advances the direction cosines linearly per spectrum:
+ * l = l0 + t * rate_l, m = m0 + t * rate_m, n = sqrt(1 - l^2 - m^2), and a
+ * window uses the direction of its first spectrum. */
+typedef struct bf_tracker_config {
+    float direction_start[3];
+    float direction_rate_per_sample[2];
+    size_t integration_spectra;
+} bf_tracker_config;
+
+/* Reference: include/beamformer/cuda_beam_tracker_v5.hpp:45-49
+ * (V5ExecutionConfig).  `kernel` picks FP32, SPLIT_F16, F16 or AUTO
+ * (= SPLIT_F16); `enable_graph` captures the batched stream's launch in a
+ * hipGraph. */
+typedef struct bf_tracker_exec {
+    int kernel;
+    int enable_graph;
+} bf_tracker_exec;
+
+/* Reference: cuda_beam_tracker_v5_device_resident
+ * (include/beamformer/cuda_beam_tracker_v5.hpp:80-85): single-beam tracker
+ * on device buffers, per-spectrum [T][F][1] output, the reference's
+ * hard-coded geometry (default_positions at 0.6 m, channels at 300 MHz +
+ * k * 300 kHz).  Validation (n_beams == 1, trajectory on the unit disk for
+ * every window) happens before launch; synchronises only if stream is NULL.
+ * Allocates nothing.  `exec` may be NULL (AUTO). */
+bf_status bf_beam_tracker_device_resident(const uint8_t* d_packed, float* d_intensity,
+                                          const bf_dims* dims,
+                                          const bf_tracker_config* tracker,
+                                          const bf_tracker_exec* exec, void* stream);
+
+/* Reference: cuda_beam_tracker_v5_into (cuda_beam_tracker_v5.hpp:74-78):
+ * host buffers in and out, synchronous. */
+bf_status bf_beam_tracker_into(const uint8_t* packed, size_t packed_bytes,
+                               const bf_dims* dims, const bf_tracker_config* tracker,
+                               float* intensity, size_t intensity_count,
+                               const bf_tracker_exec* exec);
+
+/* Reference: BatchedTrackerStreamV5 (cuda_beam_tracker_v5.hpp:90-125).
+ * `single_window_dims.n_time` is the window length; a batch holds
+ * batch_size windows.  process_batch honours first_window_index: the batch's
+ * windows are the absolute windows first_window_index .. +batch_size-1 of
+ * the trajectory (the reference ignores the index). */
+typedef struct bf_tracker_stream bf_tracker_stream;
+bf_status bf_tracker_stream_create(const bf_dims* single_window_dims,
+                                   const bf_tracker_config* tracker, size_t batch_size,
+                                   const bf_tracker_exec* exec, bf_tracker_stream** out);
+bf_status bf_tracker_stream_destroy(bf_tracker_stream* s);
+bf_status bf_tracker_stream_process_batch(bf_tracker_stream* s, size_t first_window_index,
+                                          const uint8_t* host_packed, float* host_intensity);
+bf_status bf_tracker_stream_process_batch_kernel_only(bf_tracker_stream* s,
+                                                      size_t first_window_index);
+bf_status bf_tracker_stream_last_kernel_time_ms(const bf_tracker_stream* s, float* out);
+bf_status bf_tracker_stream_device_packed_buffer(bf_tracker_stream* s, uint8_t** out);
+bf_status bf_tracker_stream_device_intensity_buffer(bf_tracker_stream* s, float** out);
+bf_status bf_tracker_stream_device_stream(bf_tracker_stream* s, void** out);
+bf_status bf_tracker_stream_sizes(const bf_tracker_stream* s, size_t* batch_size,
+                                  size_t* window_bytes, size_t* batch_voltage_bytes,
+                                  size_t* batch_output_floats);
+
+/* Multi-beam tracker (addition of this build; the reference tracker is
+ * single-beam).  Weights are regenerated per window on the device from the
+ * direction of every beam: either a device table [n_windows][n_beams][3] of
+ * unit vectors, or per-beam linear trajectories [n_beams][4] =
+ * (l0, m0, rate_l, rate_m) evaluated at absolute spectrum
+ * (window_offset + w) * integration_spectra.  Positions [n_ant][3] and
+ * frequencies [n_freq] are device tables; NULL selects the reference's
+ * default geometry (0.6 m grid, 300 MHz + k * 300 kHz).  Output is
+ * integrated [W][F][B] (integrate != 0) or per spectrum [T][F][B].
+ * n_freq may be any positive count (e.g. a full 672-channel band);
+ * n_ant in {32, 64, 128, 256}; 1 <= n_beams <= 128. */
+typedef struct bf_multibeam_args {
+    size_t n_time;
+    size_t n_freq;
+    size_t n_ant;
+    size_t n_beams;
+    size_t integration_spectra;
+    const float* positions;     /* device [n_ant][3] or NULL */
+    const float* frequencies;   /* device [n_freq] or NULL */
+    const float* directions;    /* device [W][n_beams][3] or NULL */
+    const float* trajectories;  /* device [n_beams][4] or NULL */
+    size_t window_offset;
+    int integrate;
+    int kernel;                 /* FP32, SPLIT_F16, F16 or AUTO */
+} bf_multibeam_args;
+
+bf_status bf_multibeam_tracker_launch(void* stream, const uint8_t* d_packed, float* d_out,
+                                      const bf_multibeam_args* args);
+
 /* --------------------------------------------------- device plumbing */
 /* Minimal HIP plumbing so hosts without a HIP toolchain (ctypes, JNI, cgo)
  * can own device buffers and streams.  Reference: the CUDA runtime calls
