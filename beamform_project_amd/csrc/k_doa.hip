@@ -1,0 +1,392 @@
+// k_doa.hip -- adaptive direction-of-arrival estimation for gfx950.
+//
+// Per channel f and update (K snapshots t0 .. t0+K-1):
+//   1. covariance_kernel (int8 MFMA, exact): R[r][c] = sum_k x_r conj(x_c)
+//      over the packed int4 snapshots (reference
+//      src/cpu_opt_beam_tracker.cpp:559-605 without the 1/K scale), then
+//      forward spatial smoothing Rs = sum_{l<L} R[l:l+M, l:l+M], L = A-M+1
+//      (:607-636, forward part).  Integers throughout, so bit-exact.
+//   2. fold_scan_kernel (FP64): R_block = Rs/(K L), diagonal load
+//      eps = load * tr(R_block) / M added before the fold (:244-253,
+//      :1874-1890), R = alpha R + (1-alpha) R_block (first update copies),
+//      Cholesky R = L L^H (:398-421), Q = R^-1 = L^-H L^-1, and the Bartlett
+//      a^H R a and Capon 1/(a^H Q a) scans over n_theta angles of a uniform
+//      linear array, steering a_m = exp(-j k x_m sin(theta)) (:96-106).
+//      For a ULA a^H Q a = sum_d q_d exp(j d k dx sin(theta)) with q_d the
+//      sum of Q's d-th diagonal, so each angle costs O(M), not O(M^2).
+//   3. estimate_kernel: broadband spectra summed over channels, local
+//      maxima, 3-point parabolic refinement.
+//
+// MFMA mapping: with y = [16 re, 16 im] int8 components per antenna (the
+// nibble placed in the top half of a byte is 16x the signed value), the
+// real Gram matrix G = Y^T Y (128 x 128 for A = 64) is accumulated with
+// v_mfma_i32_16x16x64_i8 over 64-snapshot steps; both operands are
+// "component x 16 snapshots" rows of a transposed LDS tile.  Then
+// Re R = (G_rr + G_ii)/256 and Im R = (G_ir - G_ri)/256 exactly.
+#include "bf_doa.hpp"
+
+namespace bfhip {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kTStride = 80;  // bytes per component row of the transposed tile
+
+// Transposes the 4 x 4 byte block of d[0..3] (rows) into e[0..3] (columns).
+__device__ __forceinline__ void transpose4x4(const unsigned d[4], unsigned e[4]) {
+    const unsigned lo01 = __builtin_amdgcn_perm(d[1], d[0], 0x05010400u);
+    const unsigned hi01 = __builtin_amdgcn_perm(d[1], d[0], 0x07030602u);
+    const unsigned lo23 = __builtin_amdgcn_perm(d[3], d[2], 0x05010400u);
+    const unsigned hi23 = __builtin_amdgcn_perm(d[3], d[2], 0x07030602u);
+    e[0] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
+    e[1] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
+    e[2] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
+    e[3] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
+}
+
+// One workgroup per channel; A in {32, 64, 128, 256} -> 2A components.
+template <int A>
+__global__ __launch_bounds__(kThreads) void covariance_kernel(
+    const uint8_t* __restrict__ x, int T, int F, int t0, int K, int M,
+    int* __restrict__ rs_out /* [F][M][M][2] */, int* __restrict__ r_full /* [F][A][A][2] or null */) {
+    constexpr int C = 2 * A;         // components
+    constexpr int NTILE = C / 16;    // tile rows/cols of G
+    constexpr int TILES = NTILE * NTILE;
+    constexpr int PER_WAVE = (TILES + 3) / 4;
+    __shared__ __attribute__((aligned(16))) unsigned char tr[C * kTStride];
+    __shared__ int g[C * (C + 1)];  // G, row stride C+1
+    const int f = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    bf_int4v acc[PER_WAVE];
+#pragma unroll
+    for (int i = 0; i < PER_WAVE; ++i) acc[i] = bf_int4v{0, 0, 0, 0};
+    const size_t row_stride = (size_t)F * A;
+    const uint8_t* xf = x + (size_t)f * A;
+    // Staging work: 64 snapshots x A antennas = 16 snapshot quads x A/4 antenna quads.
+    constexpr int QUADS = 16 * (A / 4);
+    for (int k0 = 0; k0 < K; k0 += 64) {
+        for (int q = tid; q < QUADS; q += kThreads) {
+            const int k4 = q / (A / 4);
+            const int a4 = q - k4 * (A / 4);
+            unsigned d[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int kk = k0 + 4 * k4 + i;
+                d[i] = 0u;
+                if (kk < K && t0 + kk < T)
+                    d[i] = *reinterpret_cast<const unsigned*>(
+                        xf + (size_t)(t0 + kk) * row_stride + 4 * a4);
+            }
+            unsigned e[4];
+            transpose4x4(d, e);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int a = 4 * a4 + j;
+                const unsigned re = (e[j] << 4) & 0xF0F0F0F0u;  // 16 * real
+                const unsigned im = e[j] & 0xF0F0F0F0u;         // 16 * imag
+                *reinterpret_cast<unsigned*>(tr + (2 * a) * kTStride + 4 * k4) = re;
+                *reinterpret_cast<unsigned*>(tr + (2 * a + 1) * kTStride + 4 * k4) = im;
+            }
+        }
+        __syncthreads();
+        const int rc = lane & 15;
+        const int kg = lane >> 4;
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) {
+            const int tile = wave * PER_WAVE + i;
+            if (tile < TILES) {
+                const int I = tile / NTILE;
+                const int J = tile - I * NTILE;
+                const bf_int4v av =
+                    *reinterpret_cast<const bf_int4v*>(tr + (16 * I + rc) * kTStride + 16 * kg);
+                const bf_int4v bv =
+                    *reinterpret_cast<const bf_int4v*>(tr + (16 * J + rc) * kTStride + 16 * kg);
+                acc[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc[i], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // G tiles -> LDS: lane holds column l%16, rows 4*(l/16) + r.
+#pragma unroll
+    for (int i = 0; i < PER_WAVE; ++i) {
+        const int tile = wave * PER_WAVE + i;
+        if (tile < TILES) {
+            const int I = tile / NTILE;
+            const int J = tile - I * NTILE;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                g[(16 * I + 4 * (lane >> 4) + r) * (C + 1) + 16 * J + (lane & 15)] = acc[i][r];
+        }
+    }
+    __syncthreads();
+    if (r_full != nullptr) {
+        for (int e = tid; e < A * A; e += kThreads) {
+            const int r = e / A, c = e - (e / A) * A;
+            const int gre = g[(2 * r) * (C + 1) + 2 * c] + g[(2 * r + 1) * (C + 1) + 2 * c + 1];
+            const int gim = g[(2 * r + 1) * (C + 1) + 2 * c] - g[(2 * r) * (C + 1) + 2 * c + 1];
+            r_full[(((size_t)f * A + r) * A + c) * 2] = gre / 256;
+            r_full[(((size_t)f * A + r) * A + c) * 2 + 1] = gim / 256;
+        }
+    }
+    const int L = A - M + 1;
+    for (int e = tid; e < M * M; e += kThreads) {
+        const int r = e / M, c = e - (e / M) * M;
+        int sre = 0, sim = 0;
+        for (int l = 0; l < L; ++l) {
+            const int rr = l + r, cc = l + c;
+            // every G entry is a multiple of 256 (16 x 16 scaling)
+            sre += (g[(2 * rr) * (C + 1) + 2 * cc] + g[(2 * rr + 1) * (C + 1) + 2 * cc + 1]) / 256;
+            sim += (g[(2 * rr + 1) * (C + 1) + 2 * cc] - g[(2 * rr) * (C + 1) + 2 * cc + 1]) / 256;
+        }
+        rs_out[(((size_t)f * M + r) * M + c) * 2] = sre;
+        rs_out[(((size_t)f * M + r) * M + c) * 2 + 1] = sim;
+    }
+}
+
+// ------------------------------------------------------------- FP64 stage
+struct cd {
+    double re, im;
+};
+__device__ __forceinline__ cd cmul(cd a, cd b) {
+    return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cd cmulc(cd a, cd b) {  // conj(a) * b
+    return cd{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+
+// One workgroup per channel.  buf0/buf1: M x M complex double in LDS.
+__global__ __launch_bounds__(kThreads) void fold_scan_kernel(
+    const int* __restrict__ rs, double* __restrict__ state /* [F][M][M][2] */, int first,
+    int M, int K, int L, double alpha, double load, const float* __restrict__ freqs,
+    double x_step, int n_theta, const double* __restrict__ sin_theta,
+    double* __restrict__ capon, double* __restrict__ bartlett, int* __restrict__ chol_ok) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    cd* buf0 = reinterpret_cast<cd*>(smem);
+    cd* buf1 = buf0 + M * M;
+    cd* qb = buf1 + M * M;      // [M] Bartlett diagonal sums, d = 0..M-1
+    cd* qc = qb + M;            // [M] Capon diagonal sums
+    __shared__ double s_eps;
+    __shared__ int s_ok;
+    const int f = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int MM = M * M;
+    const double scale = 1.0 / ((double)K * (double)L);
+    // R_block = Rs / (K L)
+    for (int e = tid; e < MM; e += kThreads) {
+        const int* p = rs + ((size_t)f * MM + e) * 2;
+        buf0[e] = cd{(double)p[0] * scale, (double)p[1] * scale};
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double tr = 0.0;
+        for (int i = 0; i < M; ++i) tr += buf0[i * M + i].re;
+        s_eps = load > 0.0 ? load * tr / (double)M : 0.0;
+        s_ok = 1;
+    }
+    __syncthreads();
+    // diagonal load, then fold into the carried state
+    double* st = state + (size_t)f * MM * 2;
+    for (int e = tid; e < MM; e += kThreads) {
+        cd v = buf0[e];
+        if (e / M == e % M) v.re += s_eps;
+        if (!first) {
+            v.re = alpha * st[2 * e] + (1.0 - alpha) * v.re;
+            v.im = alpha * st[2 * e + 1] + (1.0 - alpha) * v.im;
+        }
+        st[2 * e] = v.re;
+        st[2 * e + 1] = v.im;
+        buf0[e] = v;
+    }
+    __syncthreads();
+    // Bartlett diagonal sums of R: qb[d] = sum_c R[c+d][c]
+    for (int d = tid; d < M; d += kThreads) {
+        cd s{0.0, 0.0};
+        for (int c = 0; c + d < M; ++c) {
+            const cd v = buf0[(c + d) * M + c];
+            s.re += v.re;
+            s.im += v.im;
+        }
+        qb[d] = s;
+    }
+    __syncthreads();
+    // In-place right-looking Cholesky of buf0 (lower triangle).
+    for (int j = 0; j < M; ++j) {
+        if (tid == 0) {
+            const double djj = buf0[j * M + j].re;
+            if (!(djj > 0.0)) s_ok = 0;
+            buf0[j * M + j] = cd{djj > 0.0 ? sqrt(djj) : 1.0, 0.0};
+        }
+        __syncthreads();
+        const double dinv = 1.0 / buf0[j * M + j].re;
+        for (int i = j + 1 + tid; i < M; i += kThreads) {
+            cd v = buf0[i * M + j];
+            buf0[i * M + j] = cd{v.re * dinv, v.im * dinv};
+        }
+        __syncthreads();
+        const int n = M - j - 1;
+        for (int e = tid; e < n * n; e += kThreads) {
+            const int i = j + 1 + e / n;
+            const int k = j + 1 + e % n;
+            if (k <= i) {
+                const cd lik = buf0[i * M + j];
+                const cd lkj = buf0[k * M + j];
+                // A[i][k] -= L[i][j] conj(L[k][j])
+                const cd p{lik.re * lkj.re + lik.im * lkj.im, lik.im * lkj.re - lik.re * lkj.im};
+                buf0[i * M + k].re -= p.re;
+                buf0[i * M + k].im -= p.im;
+            }
+        }
+        __syncthreads();
+    }
+    const int ok = s_ok;
+    if (tid == 0) chol_ok[f] = ok;
+    // Linv (lower) into buf1, one column per thread.
+    for (int j = tid; j < M; j += kThreads) {
+        for (int i = 0; i < j; ++i) buf1[i * M + j] = cd{0.0, 0.0};
+        for (int i = j; i < M; ++i) {
+            cd s{i == j ? 1.0 : 0.0, 0.0};
+            for (int k = j; k < i; ++k) {
+                const cd p = cmul(buf0[i * M + k], buf1[k * M + j]);
+                s.re -= p.re;
+                s.im -= p.im;
+            }
+            const double inv = 1.0 / buf0[i * M + i].re;
+            buf1[i * M + j] = cd{s.re * inv, s.im * inv};
+        }
+    }
+    __syncthreads();
+    // Q = Linv^H Linv into buf0.
+    for (int e = tid; e < MM; e += kThreads) {
+        const int r = e / M, c = e % M;
+        const int k0 = r > c ? r : c;
+        cd s{0.0, 0.0};
+        for (int k = k0; k < M; ++k) {
+            const cd p = cmulc(buf1[k * M + r], buf1[k * M + c]);
+            s.re += p.re;
+            s.im += p.im;
+        }
+        buf0[e] = s;
+    }
+    __syncthreads();
+    for (int d = tid; d < M; d += kThreads) {
+        cd s{0.0, 0.0};
+        for (int c = 0; c + d < M; ++c) {
+            const cd v = buf0[(c + d) * M + c];
+            s.re += v.re;
+            s.im += v.im;
+        }
+        qc[d] = s;
+    }
+    __syncthreads();
+    // a^H X a = q_0 + 2 Re sum_{d>0} q_d e^{j d phi}, phi = k dx sin(theta).
+    const double kw = kTwoPi * (double)freqs[f] / kSpeedOfLight;
+    for (int q = tid; q < n_theta; q += kThreads) {
+        const double phi = kw * x_step * sin_theta[q];
+        double sn, cs;
+        sincos(phi, &sn, &cs);
+        const cd step{cs, sn};
+        cd e{1.0, 0.0};
+        double pb = qb[0].re, pc = qc[0].re;
+        for (int d = 1; d < M; ++d) {
+            e = cmul(e, step);
+            pb += 2.0 * (qb[d].re * e.re - qb[d].im * e.im);
+            pc += 2.0 * (qc[d].re * e.re - qc[d].im * e.im);
+        }
+        bartlett[(size_t)f * n_theta + q] = pb;
+        capon[(size_t)f * n_theta + q] = (ok && pc > 0.0) ? 1.0 / pc : 0.0;
+    }
+}
+
+// Broadband spectra (sum over channels) and peak picking, one workgroup.
+__global__ __launch_bounds__(kThreads) void estimate_kernel(
+    const double* __restrict__ capon, const double* __restrict__ bartlett, int F, int n_theta,
+    const double* __restrict__ theta, int max_peaks, double* __restrict__ broadband /* [2][n] */,
+    double* __restrict__ peaks /* [2][max_peaks][2] theta, power */,
+    int* __restrict__ n_peaks /* [2] */) {
+    const int tid = threadIdx.x;
+    for (int q = tid; q < n_theta; q += kThreads) {
+        double sc = 0.0, sb = 0.0;
+        for (int f = 0; f < F; ++f) {
+            sc += capon[(size_t)f * n_theta + q];
+            sb += bartlett[(size_t)f * n_theta + q];
+        }
+        broadband[q] = sc;
+        broadband[n_theta + q] = sb;
+    }
+    __syncthreads();
+    if (tid < 2) {
+        const double* P = broadband + tid * n_theta;
+        double* out = peaks + (size_t)tid * max_peaks * 2;
+        int n = 0;
+        for (int i = 0; i < max_peaks; ++i) out[2 * i] = out[2 * i + 1] = 0.0;
+        for (int q = 1; q + 1 < n_theta; ++q) {
+            if (!(P[q] > P[q - 1] && P[q] >= P[q + 1])) continue;
+            // parabola through (q-1, q, q+1): vertex offset in cells
+            const double pm = P[q - 1], pc = P[q], pp = P[q + 1];
+            const double den = pm - 2.0 * pc + pp;
+            double dlt = 0.0;
+            if (den < 0.0) {
+                dlt = 0.5 * (pm - pp) / den;
+                dlt = dlt > 0.5 ? 0.5 : (dlt < -0.5 ? -0.5 : dlt);
+            }
+            const double th = theta[q] + dlt * (theta[q + 1] - theta[q]);
+            // insertion into the top-max_peaks list by power
+            int pos = n < max_peaks ? n : max_peaks;
+            while (pos > 0 && out[2 * (pos - 1) + 1] < pc) {
+                if (pos < max_peaks) {
+                    out[2 * pos] = out[2 * (pos - 1)];
+                    out[2 * pos + 1] = out[2 * (pos - 1) + 1];
+                }
+                --pos;
+            }
+            if (pos < max_peaks) {
+                out[2 * pos] = th;
+                out[2 * pos + 1] = pc;
+                if (n < max_peaks) ++n;
+            }
+        }
+        n_peaks[tid] = n;
+    }
+}
+
+}  // namespace
+
+void launch_doa_covariance(hipStream_t stream, const uint8_t* x, size_t T, size_t F, size_t A,
+                           size_t t0, size_t K, size_t M, int* rs, int* r_full) {
+#define BF_COV(A_)                                                                            \
+    hipLaunchKernelGGL(covariance_kernel<A_>, dim3((unsigned)F), dim3(kThreads), 0, stream, x, \
+                       (int)T, (int)F, (int)t0, (int)K, (int)M, rs, r_full)
+    switch (A) {
+    case 32: BF_COV(32); break;
+    case 64: BF_COV(64); break;
+    default: invalid("the DOA covariance supports 32 or 64 antennas");
+    }
+#undef BF_COV
+    hip_check(hipGetLastError(), "covariance_kernel launch");
+}
+
+void launch_doa_fold_scan(hipStream_t stream, const int* rs, double* state, bool first,
+                          size_t F, size_t M, size_t K, size_t L, double alpha, double load,
+                          const float* freqs, double x_step, size_t n_theta,
+                          const double* sin_theta, double* capon, double* bartlett,
+                          int* chol_ok) {
+    const size_t smem = (2 * M * M + 2 * M) * sizeof(double) * 2;
+    hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(fold_scan_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
+              "hipFuncSetAttribute");
+    hipLaunchKernelGGL(fold_scan_kernel, dim3((unsigned)F), dim3(kThreads), smem, stream, rs,
+                       state, first ? 1 : 0, (int)M, (int)K, (int)L, alpha, load, freqs, x_step,
+                       (int)n_theta, sin_theta, capon, bartlett, chol_ok);
+    hip_check(hipGetLastError(), "fold_scan_kernel launch");
+}
+
+void launch_doa_estimate(hipStream_t stream, const double* capon, const double* bartlett,
+                         size_t F, size_t n_theta, const double* theta, size_t max_peaks,
+                         double* broadband, double* peaks, int* n_peaks) {
+    hipLaunchKernelGGL(estimate_kernel, dim3(1), dim3(kThreads), 0, stream, capon, bartlett,
+                       (int)F, (int)n_theta, theta, (int)max_peaks, broadband, peaks, n_peaks);
+    hip_check(hipGetLastError(), "estimate_kernel launch");
+}
+
+}  // namespace bfhip

@@ -336,6 +336,59 @@ typedef struct bf_multibeam_args {
 bf_status bf_multibeam_tracker_launch(void* stream, const uint8_t* d_packed, float* d_out,
                                       const bf_multibeam_args* args);
 
+/* ---------------------------------------------------- adaptive DOA */
+/* Adaptive direction finding on a uniform linear array (addition of this
+ * build; it restates the primitives of the reference's CPU DOA tracker,
+ * src/cpu_opt_beam_tracker.cpp, on the GPU).  Per update of `snapshots`
+ * spectra and per channel: exact integer covariance R = sum x x^H (int8
+ * MFMA, int32 accumulate), forward spatial smoothing over sub-arrays of
+ * `subarray` elements, R_block = R_smoothed / (K L) plus diagonal loading
+ * diagonal_load * tr(R_block) / M, exponential fold
+ * R = forgetting * R + (1 - forgetting) * R_block (the first update after
+ * create/reset copies), FP64 Cholesky, and Bartlett a^H R a and Capon
+ * 1 / (a^H R^-1 a) scans over n_theta angles uniformly spaced in
+ * [theta_min, theta_max] with a_m = exp(-j 2 pi f / c * m * spacing * sin theta).
+ * Broadband spectra sum the channels; peaks are local maxima refined with a
+ * 3-point parabola.  n_ant in {32, 64}; 1 <= subarray <= n_ant;
+ * snapshots <= 65536. */
+typedef struct bf_doa_config {
+    size_t n_freq;
+    size_t n_ant;
+    size_t snapshots;
+    size_t subarray;
+    size_t n_theta;
+    size_t max_peaks;
+    double forgetting;
+    double diagonal_load;
+    double theta_min_rad;
+    double theta_max_rad;
+    double element_spacing_m;
+    const float* frequencies; /* host [n_freq], copied at create */
+} bf_doa_config;
+
+typedef struct bf_doa bf_doa;
+bf_status bf_doa_create(const bf_doa_config* config, bf_doa** out);
+bf_status bf_doa_destroy(bf_doa* doa);
+/* The next update starts the recursion afresh (copies R_block). */
+bf_status bf_doa_reset(bf_doa* doa);
+/* Enqueues one update on `stream` over device voltages [n_time][F][A],
+ * snapshots t0 .. t0 + snapshots - 1. */
+bf_status bf_doa_update(bf_doa* doa, void* stream, const uint8_t* d_packed, size_t n_time,
+                        size_t t0);
+/* Synchronous reads of the last update.  Any pointer may be NULL.
+ * broadband: [2][n_theta] (Capon, Bartlett); peaks: [2][max_peaks][2]
+ * (theta_rad, power); n_peaks: [2]. */
+bf_status bf_doa_read_results(bf_doa* doa, double* broadband, double* peaks, int* n_peaks);
+/* Per-channel spectra [F][n_theta] each, Cholesky success flags [F]. */
+bf_status bf_doa_read_channel_spectra(bf_doa* doa, double* capon, double* bartlett,
+                                      int* chol_ok);
+/* Exact smoothed covariance of the last update, [F][M][M][2] int32. */
+bf_status bf_doa_read_covariance(bf_doa* doa, int32_t* rs);
+/* Checkpoint / resume of the recursion: R [F][M][M][2] FP64 and whether
+ * an update has been folded. */
+bf_status bf_doa_get_state(bf_doa* doa, double* state, int* initialised);
+bf_status bf_doa_set_state(bf_doa* doa, const double* state, int initialised);
+
 /* --------------------------------------------------- device plumbing */
 /* Minimal HIP plumbing so hosts without a HIP toolchain (ctypes, JNI, cgo)
  * can own device buffers and streams.  Reference: the CUDA runtime calls

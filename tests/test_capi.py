@@ -7,7 +7,7 @@ import re
 
 import pytest
 
-from beamform_project_amd import _capi, api, tracker  # noqa: F401  (registers the tracker ABI)
+from beamform_project_amd import _capi, api, doa, tracker  # noqa: F401  (register the ABI)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
