@@ -6,7 +6,11 @@
 // w[b][f][a] = (float)cos(phi), (float)sin(phi) with the phase
 // phi = (2 pi f / c) * (p_a . d) evaluated in FP64 (src/weights.cpp:37-55),
 // then contracts like the fixed grid (src/cpu_beamformer.cpp:56-78).
-// Weights are generated inside the kernels, per window, never stored.
+// Weights are generated inside the kernels, per window, never stored:
+// cos/sin are evaluated in FP64 as the reference does, except that the
+// MFMA kernels reduce the FP64 phase to [-pi, pi] and evaluate it in FP32
+// for windows shorter than 64 spectra (about one float ulp, below the
+// FP16 split's rounding), where weight generation would dominate.
 //
 // This translation unit is compiled without FMA contraction so that the
 // direction (float) and phase (double) arithmetic rounds exactly like the
@@ -25,6 +29,7 @@
 // operand serves a whole tile.  Integrated output folds the four row groups
 // of a lane column with two cross-lane adds at the window's last tile.
 #include "bf_kernels.hpp"
+#include "bf_serve.hpp"
 #include "bf_tracker.hpp"
 
 namespace bfhip {
@@ -102,34 +107,72 @@ __device__ __forceinline__ float channel_frequency(const Geom& geo, int f) {
     return geo.f_start + step;
 }
 
-// B operand of one wave for one window.
+// cos/sin of phi = k * (p . d) for short windows: the phase is formed and
+// reduced to [-pi, pi] in FP64, then evaluated in FP32 with a first-order
+// correction for the rounding of the reduced argument, so the weights
+// differ from (float)cos(phi), (float)sin(phi) by about one float ulp --
+// below the FP16 split's own rounding -- at a fraction of an FP64 sincos.
+__device__ __forceinline__ float2 steering_weight_reduced(const float p[3], const float d[3],
+                                                         double k) {
+    const double delay = (double)p[0] * d[0] + (double)p[1] * d[1] + (double)p[2] * d[2];
+    const double phase = k * delay;
+    const double turns = rint(phase * (1.0 / kTwoPi));
+    const double r = phase - turns * kTwoPi;
+    const float rh = (float)r;
+    const float rl = (float)(r - (double)rh);  // |rl| <= 2^-23
+    float sn, cs;
+    sincosf(rh, &sn, &cs);
+    // sin(rh + rl) = sin rh + rl cos rh, cos(rh + rl) = cos rh - rl sin rh
+    return make_float2(fmaf(-rl, sn, cs), fmaf(rl, cs, sn));
+}
+
+// Windows of at least this many spectra take the weights from an FP64
+// sincos, as the reference does; shorter windows, where weight generation
+// would dominate, take the reduced FP32 evaluation above.
+constexpr int kExactWeightMinSpectra = 64;
+
+// B operand of one wave for one window.  Lanes c and c^1 hold the Re and
+// Im columns of the same beam over the same antennas, so each computes
+// half of the weights and takes the other half from its partner by DPP.
 template <int A, int NT, bool SPLIT, int DIRSRC>
 __device__ __forceinline__ void window_weights(const DirArgs& da, const Geom& geo,
                                                double k, int w, int beam0, int lane,
-                                               bf_half8 (&bhi)[NT][A / 16],
+                                               bool exact, bf_half8 (&bhi)[NT][A / 16],
                                                bf_half8 (&blo)[NT][A / 16]) {
     constexpr int KS = A / 16;
     const int c = lane & 15;
     const int g = lane >> 4;
+    const int comp_im = c & 1;
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
         const int b = beam0 + nt * 8 + (c >> 1);
-        const bool comp_im = (c & 1) != 0;
         const bool live = b < da.B;
         float d[3] = {0.0f, 0.0f, 1.0f};
         if (live) beam_direction<DIRSRC>(da, w, b, d);
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-            bf_half8 h, l;
+            float2 own[2];
+            float2 other[2];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int a = g * (A / 4) + 4 * s + j;
-                float2 v = make_float2(0.0f, 0.0f);
+            for (int jj = 0; jj < 2; ++jj) {
+                const int a = g * (A / 4) + 4 * s + 2 * comp_im + jj;
+                own[jj] = make_float2(0.0f, 0.0f);
                 if (live) {
                     float p[3];
                     antenna_position(geo, a, p);
-                    v = steering_weight(p, d, k);
+                    own[jj] = exact ? steering_weight(p, d, k)
+                                    : steering_weight_reduced(p, d, k);
                 }
+            }
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj) {
+                other[jj].x = dpp_swap_pairs(own[jj].x);
+                other[jj].y = dpp_swap_pairs(own[jj].y);
+            }
+            bf_half8 h, l;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float2 v = (j >> 1) == comp_im ? own[j & 1] : other[j & 1];
                 const float e0 = comp_im ? v.y : v.x;
                 const float e1 = comp_im ? v.x : -v.y;
                 const _Float16 h0 = (_Float16)e0;
@@ -147,10 +190,37 @@ __device__ __forceinline__ void window_weights(const DirArgs& da, const Geom& ge
     }
 }
 
+// Time rows of one channel.  Spectrum t (absolute) lives at
+// x_f + (t & ring_mask) * row_stride: a linear [T][F][A] array uses
+// ring_mask = -1, a ring of 2^k spectra uses 2^k - 1.  Window w covers
+// spectra [t_base + w*S, t_base + (w+1)*S) clipped to t_limit.
+struct RowSource {
+    const uint8_t* x_f;  // x + f * A
+    size_t row_stride;   // F * A bytes
+    long ring_mask;
+    long t_base;
+    long t_limit;
+};
+
+// Output of one channel: integrated [slot][F][B] with slot = w mod
+// out_windows (0: slot = w), or per-spectrum [t - t_base][F][B].
+struct OutTarget {
+    float* out;
+    long out_windows;
+    int F;
+    int f;
+};
+
+// Windows [w_begin, w_end) of one channel, for the NT*8 beams of this wave
+// starting at beam0.  Entered by the whole workgroup (kThreads threads):
+// stages of ST window-aligned tiles of 16 spectra go through LDS in the
+// A-operand layout, the next stage is held in registers meanwhile, and the
+// B operand (this window's weights) is regenerated at each window's first
+// tile.
 template <int A, int NT, bool SPLIT, bool INTEGRATE, int DIRSRC, int ST>
-__global__ __launch_bounds__(kThreads) void tracker_mfma_kernel(
-    const uint8_t* __restrict__ x, Geom geo, DirArgs da, float* __restrict__ out, int T, int F,
-    int tpw, int n_windows, int windows_per_chunk, int n_items, int per_xcd) {
+__device__ __forceinline__ void tracker_windows(const RowSource& rs, const Geom& geo,
+                                                const DirArgs& da, const OutTarget& ot, double k,
+                                                int w_begin, int w_end, int tpw, int beam0) {
     constexpr int KS = A / 16;
     constexpr int TILE_BYTES = KS * kWaveSize * 16;
     constexpr int DW_ROW = A / 4;
@@ -161,24 +231,12 @@ __global__ __launch_bounds__(kThreads) void tracker_mfma_kernel(
 
     const int S = da.S;
     const int B = da.B;
+    const int F = ot.F;
+    const int f = ot.f;
     const int tid = threadIdx.x;
     const int lane = tid & (kWaveSize - 1);
-    const int wave = tid / kWaveSize;
-    const int item = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
-    if (item >= n_items) return;
-    const int n_chunks = n_items / F;
-    const int f = item / n_chunks;
-    const int chunk = item - f * n_chunks;
-    const int beam0 = (blockIdx.y * kWaves + wave) * NT * 8;
     const bool has_beams = beam0 < B;
-    const double k = wave_number(channel_frequency(geo, f));
-
-    const int w_begin = chunk * windows_per_chunk;
-    int w_end = w_begin + windows_per_chunk;
-    w_end = w_end < n_windows ? w_end : n_windows;
-    const int q_end = (w_end - w_begin) * tpw;  // tiles of this workgroup
-    const size_t row_stride = (size_t)F * A;
-    const uint8_t* xf = x + (size_t)f * A;
+    const int q_end = (w_end - w_begin) * tpw;  // tiles of this call
 
     unsigned pre[DW_THREAD];
     auto load_stage = [&](int q0) {
@@ -195,10 +253,10 @@ __global__ __launch_bounds__(kThreads) void tracker_mfma_kernel(
                 if (qt < q_end) {
                     const int w = w_begin + qt / tpw;
                     const int in_win = 16 * (qt % tpw) + rho;
-                    const long t = (long)w * S + in_win;
-                    if (in_win < S && t < T)
-                        v = *reinterpret_cast<const unsigned*>(xf + (size_t)t * row_stride +
-                                                               4 * j);
+                    const long t = rs.t_base + (long)w * S + in_win;
+                    if (in_win < S && t < rs.t_limit)
+                        v = *reinterpret_cast<const unsigned*>(
+                            rs.x_f + (size_t)(t & rs.ring_mask) * rs.row_stride + 4 * j);
                 }
             }
             pre[kk] = v;
@@ -242,10 +300,12 @@ __global__ __launch_bounds__(kThreads) void tracker_mfma_kernel(
 #pragma unroll 1
         for (int tl = 0; tl < ST && q0 + tl < q_end; ++tl) {
             const int qt = q0 + tl;
-            const int wl = qt / tpw;        // window within the chunk
+            const int wl = qt / tpw;        // window within this call
             const int jt = qt - wl * tpw;   // tile within the window
             const int w = w_begin + wl;
-            if (jt == 0) window_weights<A, NT, SPLIT, DIRSRC>(da, geo, k, w, beam0, lane, bhi, blo);
+            if (jt == 0)
+                window_weights<A, NT, SPLIT, DIRSRC>(da, geo, k, w, beam0, lane,
+                                                     S >= kExactWeightMinSpectra, bhi, blo);
             bf_half8 av[KS];
 #pragma unroll
             for (int s = 0; s < KS; ++s)
@@ -277,8 +337,8 @@ __global__ __launch_bounds__(kThreads) void tracker_mfma_kernel(
                         const float p = sq + dpp_swap_pairs(sq);
                         const int in_win = 16 * jt + 4 * G + r;
                         const long t = (long)w * S + in_win;
-                        if ((col & 1) == 0 && b < B && in_win < S && t < T)
-                            out[((size_t)t * F + f) * B + b] = p;
+                        if ((col & 1) == 0 && b < B && in_win < S && rs.t_base + t < rs.t_limit)
+                            ot.out[((size_t)t * F + f) * B + b] = p;
                     }
                 }
                 continue;
@@ -291,6 +351,7 @@ __global__ __launch_bounds__(kThreads) void tracker_mfma_kernel(
                 win[nt] = a;
             }
             if (jt == tpw - 1) {
+                const long slot = ot.out_windows > 0 ? (long)w % ot.out_windows : (long)w;
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
                     float v = win[nt];
@@ -299,12 +360,33 @@ __global__ __launch_bounds__(kThreads) void tracker_mfma_kernel(
                     v += dpp_swap_pairs(v);
                     const int b = beam0 + nt * 8 + (col >> 1);
                     if (G == 0 && (col & 1) == 0 && b < B)
-                        out[((size_t)w * F + f) * B + b] = v;
+                        ot.out[((size_t)slot * F + f) * B + b] = v;
                     win[nt] = 0.0f;
                 }
             }
         }
     }
+}
+
+template <int A, int NT, bool SPLIT, bool INTEGRATE, int DIRSRC, int ST>
+__global__ __launch_bounds__(kThreads) void tracker_mfma_kernel(
+    const uint8_t* __restrict__ x, Geom geo, DirArgs da, float* __restrict__ out, int T, int F,
+    int tpw, int n_windows, int windows_per_chunk, int n_items, int per_xcd) {
+    const int wave = threadIdx.x / kWaveSize;
+    const int item = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+    if (item >= n_items) return;
+    const int n_chunks = n_items / F;
+    const int f = item / n_chunks;
+    const int chunk = item - f * n_chunks;
+    const int beam0 = (blockIdx.y * kWaves + wave) * NT * 8;
+    const double k = wave_number(channel_frequency(geo, f));
+    const int w_begin = chunk * windows_per_chunk;
+    int w_end = w_begin + windows_per_chunk;
+    w_end = w_end < n_windows ? w_end : n_windows;
+    const RowSource rs{x + (size_t)f * A, (size_t)F * A, -1L, 0L, (long)T};
+    const OutTarget ot{out, 0L, F, f};
+    tracker_windows<A, NT, SPLIT, INTEGRATE, DIRSRC, ST>(rs, geo, da, ot, k, w_begin, w_end, tpw,
+                                                         beam0);
 }
 
 // ------------------------------------------------------------ FP32 tracker
@@ -392,6 +474,100 @@ __global__ __launch_bounds__(256) void tracker_fp32_kernel(
         if (threadIdx.x < BB && b0 + threadIdx.x < B)
             out[((size_t)w * F + f) * B + b0 + threadIdx.x] = red[threadIdx.x * 256];
     }
+}
+
+// ------------------------------------------------------------ serving loop
+// Plan: one thread per request slot finds the windows whose last spectrum
+// lies in (processed, head] and the items they make; a block-wide scan
+// turns the counts into item offsets.  The previous head is read before
+// the barrier and replaced after it, so each step sees its own range.
+__global__ __launch_bounds__(kServeMaxSlots) void serve_plan_kernel(
+    const ServeSlot* __restrict__ slots, ServeCtrl* __restrict__ ctrl,
+    ServeWork* __restrict__ work, int F) {
+    __shared__ int scan[kServeMaxSlots];
+    const int s = threadIdx.x;
+    const long long H = ctrl->head;
+    const long long P = ctrl->processed;
+    const ServeSlot& sl = slots[s];
+    int n_win = 0;
+    int w_lo = 0;
+    int items = 0;
+    if (sl.active) {
+        const long long S = sl.S;
+        const long long t0 = sl.t0;
+        const long long te = sl.t_end;
+        const long long W = (te - t0 + S - 1) / S;
+        // the last window may be partial and end at te, before t0 + W*S
+        const long long lo = P >= te ? W : (P <= t0 ? 0 : (P - t0) / S);
+        const long long hi = H >= te ? W : (H < t0 ? 0 : (H - t0) / S);
+        if (hi > lo) {
+            n_win = (int)(hi - lo);
+            w_lo = (int)lo;
+            items = F * ((n_win + sl.wpi - 1) / sl.wpi);
+        }
+    }
+    scan[s] = items;
+    __syncthreads();
+    for (int d = 1; d < kServeMaxSlots; d <<= 1) {
+        const int v = s >= d ? scan[s - d] : 0;
+        __syncthreads();
+        scan[s] += v;
+        __syncthreads();
+    }
+    work->off[s] = scan[s] - items;
+    work->w_lo[s] = w_lo;
+    work->n_win[s] = n_win;
+    if (s == kServeMaxSlots - 1) {
+        work->off[kServeMaxSlots] = scan[s];
+        ctrl->n_items = scan[s];
+        ctrl->processed = H;
+    }
+}
+
+// Items: persistent workgroups stride over the step's items.  Item i
+// belongs to the last slot s with off[s] <= i; within the slot it is
+// (run of windows, channel) with channels fastest.  Four waves cover up to
+// 64 beams (16 per wave).
+template <int A, bool SPLIT>
+__global__ __launch_bounds__(kThreads) void serve_items_kernel(
+    const uint8_t* __restrict__ ring, long long ring_mask, int F, Geom geo,
+    const ServeSlot* __restrict__ slots, const ServeWork* __restrict__ work) {
+    __shared__ int off[kServeMaxSlots + 1];
+    for (int i = threadIdx.x; i <= kServeMaxSlots; i += kThreads) off[i] = work->off[i];
+    __syncthreads();
+    const int n_items = off[kServeMaxSlots];
+    const int wave = threadIdx.x / kWaveSize;
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+        int lo = 0;
+        int hi = kServeMaxSlots;  // off[lo] <= item < off[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (off[mid] <= item)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const ServeSlot* sl = slots + lo;
+        const int local = item - off[lo];
+        const int run = local / F;
+        const int f = local - run * F;
+        const int S = sl->S;
+        const int w_first = work->w_lo[lo];
+        const int w_last = w_first + work->n_win[lo];
+        const int w_begin = w_first + run * sl->wpi;
+        const int w_end = w_begin + sl->wpi < w_last ? w_begin + sl->wpi : w_last;
+        const DirArgs da{nullptr, &sl->traj[0][0], 0L, S, sl->B};
+        const RowSource rs{ring + (size_t)f * A, (size_t)F * A, (long)ring_mask, (long)sl->t0,
+                           (long)sl->t_end};
+        const OutTarget ot{sl->out, (long)sl->out_windows, F, f};
+        const double k = wave_number(channel_frequency(geo, f));
+        tracker_windows<A, 2, SPLIT, true, kDirTrajectory, 4>(rs, geo, da, ot, k, w_begin, w_end,
+                                                              (S + 15) / 16, wave * 16);
+    }
+}
+
+__global__ void serve_set_head_kernel(ServeCtrl* ctrl, long long head) {
+    if (threadIdx.x == 0) ctrl->head = head;
 }
 
 template <int A, int NT, bool SPLIT, bool INTEGRATE, int DIRSRC>
@@ -483,6 +659,31 @@ void launch_tracker(hipStream_t stream, const TrackerLaunch& L) {
     else
         dispatch<kDirTrajectory>(stream, L, da);
     hip_check(hipGetLastError(), "tracker kernel launch");
+}
+
+void launch_serve_set_head(hipStream_t stream, ServeCtrl* ctrl, long long head) {
+    hipLaunchKernelGGL(serve_set_head_kernel, dim3(1), dim3(64), 0, stream, ctrl, head);
+    hip_check(hipGetLastError(), "serve head launch");
+}
+
+void launch_serve_step(hipStream_t stream, const ServeLaunch& L) {
+    hipLaunchKernelGGL(serve_plan_kernel, dim3(1), dim3(kServeMaxSlots), 0, stream, L.slots,
+                       L.ctrl, L.work, (int)L.F);
+#define BF_SERVE(A_, SPLIT_)                                                                 \
+    hipLaunchKernelGGL((serve_items_kernel<A_, SPLIT_>), dim3((unsigned)L.grid), dim3(kThreads), \
+                       0, stream, L.ring, L.ring_mask, (int)L.F, L.geo, L.slots, L.work)
+    if (L.A == 32 && L.split)
+        BF_SERVE(32, true);
+    else if (L.A == 32)
+        BF_SERVE(32, false);
+    else if (L.A == 64 && L.split)
+        BF_SERVE(64, true);
+    else if (L.A == 64)
+        BF_SERVE(64, false);
+    else
+        invalid("the serving loop supports 32 or 64 antennas");
+#undef BF_SERVE
+    hip_check(hipGetLastError(), "serve step launch");
 }
 
 }  // namespace bfhip

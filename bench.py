@@ -11,9 +11,15 @@ split-FP16 MFMA beamform + |.|^2 + 10-spectrum integration (f32 output)
 followed by the int8 quantiser (codes + parameters), inputs resident in HBM.
 
 Prints one JSON line (see the keys below).  The plain FP32 fused kernel is
-timed beside it for the speedup figure.  `cpu_baseline` times the reference
-CPU path (oracle/_ref, compiled from the reference's sources; the restated
-oracle when _ref is absent) on a bounded sample of the same workload.
+timed beside it for the speedup figure; `max_rel_err_vs_fp64` compares the
+first 32 integrated windows with an FP64 (numpy complex128) evaluation of
+the same sums; `serving` runs the streaming tracker server under Poisson
+request arrivals at the stream's real-time rate (A=64, F=336, ring 2^20
+spectra, 1-64 sources, 10- or 320-spectrum windows, 0.5-5 s requests) and
+reports per-window latency percentiles and processed beam-samples/s.
+`cpu_baseline` times the reference CPU path (oracle/_ref, compiled from the
+reference's sources; the restated oracle when _ref is absent) on a bounded
+sample of the same workload.
 """
 from __future__ import annotations
 
@@ -36,9 +42,11 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="c2", choices=["c2", "c1"])
+    p.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "b1"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--no-serving", action="store_true")
+    p.add_argument("--serve-seconds", type=float, default=6.0)
     return p.parse_args()
 
 
@@ -91,6 +99,10 @@ def main():
         sys.exit(2)
     from beamform_project_amd import api
 
+    if args.workload in ("c3", "b1", "c4"):
+        line = {"c3": bench_c3, "b1": bench_b1, "c4": bench_c4}[args.workload](args)
+        print(json.dumps(line), flush=True)
+        return
     wl = workload(args.workload)
     A, T, B, S = wl["A"], wl["T"], wl["B"], wl["S"]
     dims = api.Dimensions(T, F, A, B)
@@ -178,6 +190,14 @@ def main():
     roof["kernel_avg_ms"] = round(kern_ms, 4)
     roof["algorithmic_bytes"] = bytes_alg
 
+    err = None
+    if S:
+        err = fp64_error(x, out, w, A, B, S)
+
+    serving = None
+    if not args.no_serving:
+        serving = serving_run(args.serve_seconds)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(wl, pos, fr, dirs, args.cpu_seconds)
@@ -198,8 +218,11 @@ def main():
         "config": {"workload": wl["name"], "n_ant": A, "n_freq": F, "n_time": T,
                    "n_beams": B, "integration_spectra": S, "kernel": wl["kernel"],
                    "outputs": "f32 integrated + int8 codes/params" if S else "f32"},
+        "max_rel_err_vs_fp64": err,
+        "p99_window_latency_ms": serving["latency_p99_ms"] if serving else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serving,
         "fp32_fused_kernel": {"ms_per_step": round(fp32_ms, 4),
                               "kernel_avg_ms": round(fp32_kern, 4),
                               "beam_samples_per_s": round(beam_samples / (fp32_ms * 1e-3), 1),
@@ -207,6 +230,258 @@ def main():
                               "split_over_fp32_kernel": round(fp32_kern / split_kern, 2)},
     }
     print(json.dumps(line), flush=True)
+
+
+def common_line(args, metric, value, unit, ms, dtype, config, **extra):
+    line = {"metric": metric, "value": round(value, 1), "unit": unit, "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic (device-generated, seeded)", "config": config}
+    line.update(extra)
+    return line
+
+
+def timed_launches(stream, launch, steps, warmup):
+    """Average wall ms per launch and average HIP-event ms of the launch."""
+    from beamform_project_amd import api
+    e0, e1 = api.Event(), api.Event()
+    for _ in range(warmup):
+        launch()
+    stream.synchronize()
+    kern = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e0.record(stream)
+        launch()
+        e1.record(stream)
+        kern.append(e0.elapsed_ms(e1))
+    stream.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, float(np.mean(kern))
+
+
+def tracker_fp64_error(x_dev, out, pos, fr, dirs, T0, F_, A, B, S):
+    """Floored max relative error of the first T0/S integrated windows of
+    the tracker against complex128 sums with weights (float)cos/sin of the
+    FP64 phase of each window's direction."""
+    from beamform_project_amd import _capi
+    W0 = T0 // S
+    xb = np.empty((T0, F_, A), np.uint8)
+    _capi.call("bf_memcpy_dtoh", xb.ctypes.data, x_dev.ptr, xb.nbytes)
+    got = out.to_numpy()[:W0]
+    re = ((xb & 0xF).astype(np.int16) ^ 8) - 8
+    im = ((xb >> 4).astype(np.int16) ^ 8) - 8
+    xc = (re.astype(np.float64) + 1j * im.astype(np.float64)).transpose(1, 0, 2)  # [F][T][A]
+    k = 2 * np.pi * fr.astype(np.float64) / 299792458.0
+    ref = np.zeros((W0, F_, B))
+    for w in range(W0):
+        delay = dirs[w].astype(np.float64) @ pos.astype(np.float64).T  # [B][A]
+        ph = k[:, None, None] * delay[None]  # [F][B][A]
+        wc = np.cos(ph).astype(np.float32).astype(np.float64) + \
+            1j * np.sin(ph).astype(np.float32).astype(np.float64)
+        v = np.matmul(xc[:, w * S:(w + 1) * S], wc.transpose(0, 2, 1))  # [F][S][B]
+        ref[w] = (v.real ** 2 + v.imag ** 2).sum(1)
+    floor = A * float(np.mean(re.astype(np.float64) ** 2 + im.astype(np.float64) ** 2)) * S
+    return float(np.max(np.abs(got - ref) / np.maximum(ref, floor)))
+
+
+def bench_c3(args):
+    """Dynamic multi-source tracker over the full band on one GPU."""
+    from beamform_project_amd import api, tracker as TR
+    A, F_, T, S, B = 64, 672, 65536, 320, 32
+    W = (T + S - 1) // S
+    c, r = np.meshgrid(np.arange(8), np.arange(8))
+    pos = np.stack([0.3 * c.ravel(), 0.3 * r.ravel(), np.zeros(64)], 1).astype(np.float32)
+    fr = np.linspace(400e6, 800e6, F_).astype(np.float32)
+    rng = np.random.default_rng(33)
+    az0 = rng.uniform(0, 2 * np.pi, B)
+    el0 = np.deg2rad(rng.uniform(30, 90, B))
+    rate = np.deg2rad(rng.uniform(0, 15, (2, B))) / 3600.0  # rad/s
+    tw = np.arange(W) * S * 2.56e-6
+    az = az0[None] + rate[0][None] * tw[:, None]
+    el = np.minimum(el0[None] + rate[1][None] * tw[:, None], np.pi / 2)
+    dirs = np.stack([np.cos(el) * np.sin(az), np.cos(el) * np.cos(az), np.sin(el)],
+                    -1).astype(np.float32)  # [W][B][3]
+    stream = api.Stream()
+    x = api.DeviceArray.empty((T, F_, A), np.uint8)
+    dpos, dfr, ddir = (api.DeviceArray.from_numpy(a) for a in (pos, fr, dirs))
+    sigma = 2.5
+    api.generate_point_source_noise(stream, x, T, F_, A, dpos, dfr, dirs[0, 0],
+                                    float(np.sqrt(3.0 * 2 * sigma * sigma)), sigma, 303)
+    out = api.DeviceArray.empty((W, F_, B), np.float32)
+    stream.synchronize()
+    res = {}
+    for name in ("SPLIT_F16", "F16", "FP32"):
+        kern = api.Kernel[name]
+
+        def launch():
+            TR.multibeam_tracker_launch(stream, x, out, T, F_, A, B, S, dpos, dfr,
+                                        directions=ddir, kernel=kern)
+        steps = args.steps if name != "FP32" else max(3, args.steps // 4)
+        ms, kms = timed_launches(stream, launch, steps, args.warmup)
+        err = tracker_fp64_error(x, out, pos, fr, dirs, 2 * S, F_, A, B, S)
+        res[name] = {"ms_per_step": round(ms, 4), "kernel_avg_ms": round(kms, 4),
+                     "beam_samples_per_s": round(T * F_ * B / (ms * 1e-3), 1),
+                     "max_rel_err_vs_fp64": err}
+    split = res["SPLIT_F16"]
+    cmac = T * F_ * B * A
+    flops = 8 * cmac + 4 * T * F_ * B
+    ach = flops / (split["kernel_avg_ms"] * 1e-3) / 1e12
+    cpu = None if args.no_cpu_baseline else tracker_cpu_baseline(args.cpu_seconds, B, F_)
+    return common_line(
+        args, "beam-samples/s (T·F·B)", T * F_ * B / (split["ms_per_step"] * 1e-3),
+        "beam-samples/s", split["ms_per_step"], "f16 split hi+lo, f32 accumulate",
+        {"workload": "C3 dynamic tracker: A=64 8x8 0.3 m, F=672 (400-800 MHz), T=65536, "
+                     "S=320 (W=205), B=32 sources drifting 0-15 deg/h in az/el, weights "
+                     "regenerated per window on the device, point source + noise sigma 2.5",
+         "n_ant": A, "n_freq": F_, "n_time": T, "n_beams": B, "integration_spectra": S,
+         "kernel": "SPLIT_F16"},
+        max_rel_err_vs_fp64=split["max_rel_err_vs_fp64"],
+        roofline={"bound": "mfma", "achieved": round(ach, 2), "peak": 2500.0,
+                  "unit": "TFLOP/s", "frac": round(ach / 2500.0, 4), "traffic": None,
+                  "kernel_avg_ms": split["kernel_avg_ms"],
+                  "algorithmic_bytes": T * F_ * A + W * F_ * B * 4},
+        cpu_baseline=cpu, paths=res)
+
+
+def tracker_cpu_baseline(seconds, B, F_):
+    """The reference's OpenMP CPU tracker (beam_tracker_opt_v2, single beam,
+    F=336 as it requires) on all host cores over a bounded T sample, scaled
+    to beam-samples/s of B beams over F_ channels."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker/baseline library, never the product path
+    R = O.ref_lib()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    A, F0, S = 64, 336, 320
+
+    def run(T):
+        x = O.make_uniform_int4(T * F0 * A, 9)
+        out = np.zeros((T, F0, 1), np.float32)
+        t0 = time.perf_counter()
+        if R is not None:
+            R.ref_beam_tracker_opt_v2(x, T, F0, A, 0.0, 0.0, 1e-5, 0.0, S, out)
+        else:
+            O.beam_tracker(x, T, F0, A, 0.0, 0.0, 1e-5, 0.0, S, threads=True)
+        return time.perf_counter() - t0
+
+    dt = run(S)
+    T = max(S, int(S * seconds / 3 / max(dt, 1e-6)) // S * S)
+    med = float(np.median([run(T) for _ in range(3)]))
+    return {"value": round(T * F0 / med, 1), "unit": "beam-samples/s", "cores": threads,
+            "kind": "reference" if R is not None else "port",
+            "sample": f"beam_tracker_opt_v2 (OpenMP), one beam, T={T} x F={F0}, A={A}, "
+                      f"S={S}; median of 3 runs of {med:.2f} s; the C3 job is B={B} such "
+                      f"beams over F={F_} channels"}
+
+
+def bench_b1(args):
+    """The reference's published tracker case: one beam, A=64, T=15360,
+    F=336, S=320, device-resident (weights regenerated on the device)."""
+    from beamform_project_amd import api, tracker as TR
+    A, F_, T, S = 64, 336, 15360, 320
+    x = api.DeviceArray.empty((T, F_, A), np.uint8)
+    stream = api.Stream()
+    api.generate_uniform_int4(stream, x, x.nbytes, 77)
+    out = api.DeviceArray.empty((T, F_, 1), np.float32)
+    stream.synchronize()
+    dims = api.Dimensions(T, F_, A, 1)
+    cfg = TR.TrackerConfig(TR.direction_from_lm(0.0, 0.0), (1e-5, 0.0), S)
+    res = {}
+    for name in ("SPLIT_F16", "FP32"):
+        ex = TR.TrackerExec(api.Kernel[name])
+
+        def launch():
+            TR.beam_tracker_device_resident(x, out, dims, cfg, ex, stream)
+        ms, kms = timed_launches(stream, launch, args.steps, args.warmup)
+        res[name] = {"ms_per_step": round(ms, 4), "kernel_avg_ms": round(kms, 4),
+                     "beam_samples_per_s": round(T * F_ / (ms * 1e-3), 1)}
+    best = min(res, key=lambda k: res[k]["ms_per_step"])
+    ms = res[best]["ms_per_step"]
+    value = T * F_ / (ms * 1e-3)
+    return common_line(
+        args, "beam-samples/s (T·F·B)", value, "beam-samples/s", ms, "f32" if best == "FP32"
+        else "f16 split hi+lo, f32 accumulate",
+        {"workload": "B=1 dynamic tracker, device-resident: A=64 (reference grid 0.6 m), "
+                     "F=336, T=15360, S=320, per-spectrum output, uniform int4",
+         "n_ant": A, "n_freq": F_, "n_time": T, "n_beams": 1, "integration_spectra": S,
+         "kernel": best},
+        vs_baseline=round(value / 3.59e8, 2),
+        baseline_note="reference CUDA V5 device-resident A=64: 14.38 ms = 3.59e8 "
+                      "beam-samples/s on a Quadro P1000",
+        paths=res)
+
+
+def bench_c4(args):
+    """The streaming server alone (serving leg of the default line)."""
+    sv = serving_run(args.serve_seconds)
+    return common_line(
+        args, "beam-samples/s (T·F·B); p99 per-window latency", sv["beam_samples_per_s"],
+        "beam-samples/s", sv["wall_seconds"] * 1e3, "f16 split hi+lo, f32 accumulate",
+        {"workload": sv["workload"]}, p99_window_latency_ms=sv["latency_p99_ms"], serving=sv)
+
+
+def fp64_error(x, out, w, A, B, S, n_windows=32):
+    """Floored max relative error of the first n_windows integrated windows
+    against the same sums in FP64 (f32 weights, complex128 arithmetic);
+    floor = A * E|x|^2 * S, the incoherent per-beam power."""
+    from beamform_project_amd import _capi
+    T = n_windows * S
+    xb = np.empty((T, F, A), np.uint8)
+    _capi.call("bf_memcpy_dtoh", xb.ctypes.data, x.ptr, xb.nbytes)
+    got = np.empty((n_windows, F, B), np.float32)
+    _capi.call("bf_memcpy_dtoh", got.ctypes.data, out.ptr, got.nbytes)
+    re = ((xb & 0xF).astype(np.int16) ^ 8) - 8
+    im = ((xb >> 4).astype(np.int16) ^ 8) - 8
+    xc = re.astype(np.float64) + 1j * im.astype(np.float64)  # [T][F][A]
+    wc = w[..., 0].astype(np.float64) + 1j * w[..., 1].astype(np.float64)  # [B][F][A]
+    v = np.matmul(xc.transpose(1, 0, 2), wc.transpose(1, 2, 0))  # [F][T][B]
+    p = (v.real ** 2 + v.imag ** 2).reshape(F, n_windows, S, B).sum(2).transpose(1, 0, 2)
+    floor = A * float(np.mean(np.abs(xc) ** 2)) * S
+    return float(np.max(np.abs(got - p) / np.maximum(p, floor)))
+
+
+def serving_run(seconds):
+    """The streaming server under Poisson arrivals, paced at 2.56 us per
+    spectrum (the native load generator bf_server_simulate)."""
+    from beamform_project_amd import serve as SV
+
+    c, r = np.meshgrid(np.arange(8), np.arange(8))
+    pos = np.stack([0.3 * c.ravel(), 0.3 * r.ravel(), np.zeros(64)], 1).astype(np.float32)
+    fr = np.linspace(400e6, 800e6, F).astype(np.float32)
+    cfg = SV.SimConfig(
+        server=SV.ServerConfig(n_freq=F, n_ant=64, ring_spectra=1 << 20, max_window=320,
+                               max_step_spectra=8192, hbm_budget_bytes=2 << 30,
+                               max_requests=256, positions=pos, frequencies=fr,
+                               enable_graph=True),
+        seed=4242, cadence_s=2.56e-6, run_seconds=seconds, chunk_spectra=128,
+        arrival_rate_hz=6.0, min_beams=1, max_beams=64, min_duration_s=0.5,
+        max_duration_s=5.0, window_a=10, window_b=320, output_windows_cap=1024, paced=True,
+        pool_spectra=65536, source_amplitude=float(np.sqrt(3.0 * 2 * 2.5 * 2.5)),
+        noise_sigma=2.5)
+    rep = SV.simulate(cfg)
+    st = rep["stats"]
+    return {
+        "workload": "C4 streaming: Poisson arrivals 6/s, sources U{1..64}, windows 10|320, "
+                    "durations U[0.5,5] s, A=64, F=336, ring 2^20 spectra (22.5 GB), "
+                    "HBM budget 2 GiB for outputs (1024-window output rings), "
+                    f"{seconds:g} s of stream paced at 2.56 us/spectrum, 128-spectrum ingest",
+        "latency_p50_ms": round(st["latency_p50_ms"], 4),
+        "latency_p90_ms": round(st["latency_p90_ms"], 4),
+        "latency_p99_ms": round(st["latency_p99_ms"], 4),
+        "latency_max_ms": round(st["latency_max_ms"], 4),
+        "latency_budget_ms": 0.5,
+        "windows_timed": st["windows_timed"],
+        "beam_samples_per_s": round(rep["beam_samples_per_s"], 1),
+        "gpu_busy_frac": round(st["gpu_busy_ms"] / 1e3 / rep["wall_seconds"], 4),
+        "steps": st["steps"],
+        "requests_generated": rep["requests_generated"],
+        "requests_admitted": st["requests_admitted"],
+        "requests_completed": st["requests_completed"],
+        "mean_requests_in_slots": round(rep["mean_requests_in_slots"], 2),
+        "arena_peak_bytes": st["arena_peak_bytes"],
+        "mean_admission_wait_ms": round(st["mean_admission_wait_spectra"] * 2.56e-3, 3),
+        "max_pacing_lag_ms": round(rep["max_pacing_lag_ms"], 3),
+        "wall_seconds": round(rep["wall_seconds"], 3),
+    }
 
 
 def read_traffic(wl):

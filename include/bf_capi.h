@@ -389,6 +389,164 @@ bf_status bf_doa_read_covariance(bf_doa* doa, int32_t* rs);
 bf_status bf_doa_get_state(bf_doa* doa, double* state, int* initialised);
 bf_status bf_doa_set_state(bf_doa* doa, const double* state, int initialised);
 
+/* ------------------------------------------- streaming tracker server */
+/* Continuous-batching server for tracking requests over one voltage stream
+ * (addition of this build; a superset of the reference's
+ * BatchedTrackerStreamV5, include/beamformer/cuda_beam_tracker_v5.hpp:90-125,
+ * which runs fixed batches of one request synchronously).
+ *
+ * Voltages [n][F][A] are ingested into a device ring of ring_spectra
+ * spectra (spectrum t at row t mod ring_spectra).  A request tracks
+ * n_beams beams along linear trajectories [n_beams][4] =
+ * (l0, m0, rate_l, rate_m), evaluated at each window's first spectrum
+ * counted from the request's start, over n_spectra spectra in windows of
+ * `window` spectra (the last may be partial), and produces integrated
+ * power [window][F][n_beams] f32 -- the multi-beam tracker's output for the
+ * same span -- into an output ring of output_windows windows in the
+ * server's HBM arena.  Requests wait in first-in first-out order until a
+ * request slot and arena memory are free (admission by HBM budget).  Each
+ * step processes, for every admitted request, the windows whose last
+ * spectrum was ingested since the previous step, in one captured launch
+ * (hipGraph when enable_graph).  Per-window latency runs from the ingest
+ * copy that landed a window's last spectrum to the end of the step that
+ * produced it (device events).  One host thread drives a server; calls
+ * that overlap on one server race. */
+typedef struct bf_server bf_server;
+
+typedef struct bf_server_config {
+    size_t n_freq;            /* channels F */
+    size_t n_ant;             /* 32 or 64 */
+    size_t ring_spectra;      /* power of two */
+    size_t max_window;        /* longest window accepted, <= ring_spectra / 2 */
+    size_t max_step_spectra;  /* most spectra one step releases */
+    size_t hbm_budget_bytes;  /* arena for request outputs */
+    size_t max_requests;      /* request slots, 1..256 */
+    const float* positions;   /* host [n_ant][3] m, copied; NULL: reference grid */
+    const float* frequencies; /* host [n_freq] Hz, copied; NULL: reference plan */
+    int kernel;               /* SPLIT_F16, F16 or AUTO (= SPLIT_F16) */
+    int enable_graph;
+} bf_server_config;
+
+typedef struct bf_server_request {
+    size_t n_beams;              /* 1..64 */
+    size_t window;               /* 1..max_window */
+    size_t n_spectra;            /* duration */
+    long long start;             /* first spectrum; -1: the head at admission */
+    size_t output_windows;       /* output ring depth; 0: every window */
+    const float* trajectories;   /* host [n_beams][4], copied */
+} bf_server_request;
+
+enum bf_server_request_state {
+    BF_REQUEST_PENDING = 0,
+    BF_REQUEST_ACTIVE = 1,
+    BF_REQUEST_COMPLETE = 2,
+    BF_REQUEST_REJECTED = 3 /* its start left the ring before admission */
+};
+
+typedef struct bf_request_status {
+    int state;
+    long long start;          /* first spectrum once admitted */
+    size_t windows_total;
+    size_t windows_issued;    /* windows handed to a step */
+    size_t windows_done;      /* windows whose outputs are complete */
+    size_t output_windows;
+    size_t output_bytes;
+} bf_request_status;
+
+typedef struct bf_server_stats {
+    size_t windows_timed;
+    double latency_p50_ms;
+    double latency_p90_ms;
+    double latency_p99_ms;
+    double latency_max_ms;
+    double latency_mean_ms;
+    size_t steps;
+    size_t windows_processed;
+    double beam_samples;      /* sum over processed windows of spectra * F * B */
+    double gpu_busy_ms;       /* sum of step durations */
+    size_t requests_submitted;
+    size_t requests_admitted;
+    size_t requests_completed;
+    size_t requests_rejected;
+    size_t requests_pending;
+    size_t requests_in_slots;
+    size_t arena_bytes_in_use;
+    size_t arena_peak_bytes;
+    long long head;           /* spectra ingested */
+    long long processed_head; /* spectra released to steps */
+    double mean_admission_wait_spectra;
+} bf_server_stats;
+
+bf_status bf_server_create(const bf_server_config* config, bf_server** out);
+bf_status bf_server_destroy(bf_server* server);
+/* Queues a request; it is admitted by a later bf_server_step. */
+bf_status bf_server_submit(bf_server* server, const bf_server_request* request,
+                           uint64_t* id);
+/* Appends n_spectra spectra [n][F][A] (device memory when src_on_device,
+ * else host) to the ring: asynchronous for a device source, complete on
+ * return for a host source.  Fails with BF_ERR_LOGIC when it would
+ * overwrite spectra of windows no step has processed yet. */
+bf_status bf_server_ingest(bf_server* server, const uint8_t* src, size_t n_spectra,
+                           int src_on_device);
+/* Admits what fits, then enqueues one step over the spectra ingested since
+ * the previous step (at most max_step_spectra); returns the windows it
+ * scheduled.  Asynchronous. */
+bf_status bf_server_step(bf_server* server, size_t* windows_scheduled);
+bf_status bf_server_request_status(bf_server* server, uint64_t id,
+                                   bf_request_status* status);
+/* Copies windows [first, first + n) of a request, [n][F][n_beams] f32;
+ * waits for the step that produced them. */
+bf_status bf_server_read_output(bf_server* server, uint64_t id, size_t first_window,
+                                size_t n_windows, float* host_out);
+/* Frees a request's slot and outputs (cancels it if still running). */
+bf_status bf_server_release(bf_server* server, uint64_t id);
+bf_status bf_server_synchronize(bf_server* server);
+bf_status bf_server_get_stats(bf_server* server, bf_server_stats* stats);
+bf_status bf_server_reset_stats(bf_server* server);
+
+/* Native load generator: Poisson request arrivals at arrival_rate_hz over
+ * run_seconds of stream time, each request with n_beams uniform in
+ * [min_beams, max_beams], window window_a or window_b with equal odds and
+ * duration uniform in [min_duration_s, max_duration_s]; trajectories start
+ * uniform in |l|, |m| <= 0.5 and drift at most 15 degrees per hour.
+ * Voltages are a device pool of pool_spectra spectra (one point source
+ * plus noise) cycled into the ring chunk_spectra at a time, at the
+ * stream's real-time rate when paced (else as fast as the steps run);
+ * every chunk is followed by the steps it releases.  Finished requests are
+ * released. */
+typedef struct bf_server_sim_config {
+    bf_server_config server;
+    uint64_t seed;
+    double cadence_s;
+    double run_seconds;
+    size_t chunk_spectra;
+    double arrival_rate_hz;
+    size_t min_beams;
+    size_t max_beams;
+    double min_duration_s;
+    double max_duration_s;
+    size_t window_a;
+    size_t window_b;
+    size_t output_windows_cap;
+    int paced;
+    size_t pool_spectra;
+    float source_amplitude;
+    float noise_sigma;
+} bf_server_sim_config;
+
+typedef struct bf_server_sim_report {
+    bf_server_stats stats;
+    double wall_seconds;
+    double stream_seconds;
+    double beam_samples_per_s;     /* processed beam-samples / wall time */
+    double mean_requests_in_slots;
+    size_t requests_generated;
+    double max_pacing_lag_ms;
+    int graph_used;
+} bf_server_sim_report;
+
+bf_status bf_server_simulate(const bf_server_sim_config* config, bf_server_sim_report* report);
+
 /* --------------------------------------------------- device plumbing */
 /* Minimal HIP plumbing so hosts without a HIP toolchain (ctypes, JNI, cgo)
  * can own device buffers and streams.  Reference: the CUDA runtime calls

@@ -7,7 +7,7 @@ import re
 
 import pytest
 
-from beamform_project_amd import _capi, api, doa, tracker  # noqa: F401  (register the ABI)
+from beamform_project_amd import _capi, api, doa, serve, tracker  # noqa: F401  (register the ABI)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -90,3 +90,14 @@ def test_error_message_is_thread_local_and_cleared():
     assert b"n_time" in _capi.LIB.bf_last_error_message()
     api.validate_dimensions(api.Dimensions(1, 336, 64, 1))
     assert _capi.LIB.bf_last_error_message() == b""
+
+
+def test_server_config_is_validated_before_any_device_call():
+    base = dict(n_freq=336, n_ant=64, ring_spectra=1 << 12, max_window=320,
+                max_step_spectra=4096, hbm_budget_bytes=1 << 20, max_requests=8)
+    for bad, msg in ((dict(n_ant=128), "antennas"), (dict(ring_spectra=1000), "power of two"),
+                     (dict(max_window=4096), "max_window"), (dict(max_requests=300), "max_requests"),
+                     (dict(kernel=api.Kernel.FP32), "MFMA")):
+        cfg = serve.ServerConfig(**{**base, **bad})
+        with pytest.raises(_capi.InvalidArgument, match=msg):
+            serve.TrackerServer(cfg)

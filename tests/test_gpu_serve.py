@@ -1,0 +1,242 @@
+"""GPU tests of the streaming tracker server.
+
+Bars: a request's integrated output [W][F][B] is bit-identical whether it
+runs alone or mixed with other requests, and to the multi-beam tracker
+(same kernel arithmetic) over the same span of voltages; against the FP64
+oracle it holds the split-FP16 bar (floored relative error <= 1e-5).
+The ring is kept small so that windows wrap around it.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from beamform_project_amd import _capi, api, serve as SV, tracker as TR
+from helpers import F, incoherent_floor, rel_err_floored
+
+pytestmark = pytest.mark.gpu
+K = api.Kernel
+A = 64
+POS = O.regular_array(8, 8, 0.3)
+FR = np.linspace(400e6, 800e6, F).astype(np.float32)
+
+
+def trajectories(B, seed):
+    rng = np.random.default_rng(seed)
+    t = np.zeros((B, 4), np.float32)
+    t[:, 0] = rng.uniform(-0.4, 0.4, B)
+    t[:, 1] = rng.uniform(-0.4, 0.4, B)
+    t[:, 2:] = rng.uniform(-2e-5, 2e-5, (B, 2))
+    return t
+
+
+def voltages(T, seed):
+    return O.make_uniform_int4(T * F * A, seed).reshape(T, F, A)
+
+
+def config(**kw):
+    base = dict(n_freq=F, n_ant=A, ring_spectra=1024, max_window=320, max_step_spectra=4096,
+                hbm_budget_bytes=64 << 20, max_requests=16, positions=POS, frequencies=FR,
+                enable_graph=True)
+    base.update(kw)
+    return SV.ServerConfig(**base)
+
+
+# (B, S, n_spectra, start, submit_at, trajectory seed): start -1 = the head
+# at admission.
+MIX = [
+    (5, 10, 1234, 0, 0, 10),
+    (64, 320, 960, 100, 0, 11),
+    (17, 16, 1000, 700, 300, 12),
+    (1, 320, 700, -1, 1200, 13),
+    (33, 10, 333, 1500, 1500, 14),
+]
+
+
+def run(requests, x, chunk=100, **kw):
+    """Streams x through a server, submitting request i once `submit_at`
+    spectra have been ingested; returns {i: (start, output)}."""
+    srv = SV.TrackerServer(config(**kw))
+    ids = {}
+    T = x.shape[0]
+    for h in range(0, T, chunk):
+        for i, (B, S, n, start, at, seed) in enumerate(requests):
+            if at == h:
+                ids[i] = srv.submit(SV.Request(trajectories(B, seed), S, n, start))
+        srv.ingest(x[h:h + chunk])
+        srv.step()
+    srv.synchronize()
+    out = {}
+    for i, rid in ids.items():
+        st = srv.status(rid)
+        B = requests[i][0]
+        assert st["state"] == SV.COMPLETE, (i, st)
+        out[i] = (st["start"], srv.read(rid, 0, st["windows_total"], B))
+        srv.release(rid)
+    stats = srv.stats()
+    srv.close()
+    return out, stats
+
+
+@pytest.fixture(scope="module")
+def stream():
+    return voltages(2400, 99)
+
+
+@pytest.fixture(scope="module")
+def mixed(stream):
+    return run(MIX, stream)
+
+
+def test_alone_equals_mixed(stream, mixed):
+    out, _ = mixed
+    for i, req in enumerate(MIX):
+        start = out[i][0]
+        alone, _ = run([(req[0], req[1], req[2], start, req[4], req[5])], stream)
+        assert alone[0][0] == start
+        assert np.array_equal(alone[0][1], out[i][1]), f"request {i}"
+
+
+def test_matches_multibeam_tracker_and_fp64(stream, mixed):
+    out, stats = mixed
+    for i, (B, S, n, _, _, seed) in enumerate(MIX):
+        t0, got = out[i]
+        x = np.ascontiguousarray(stream[t0:t0 + n])
+        assert x.shape[0] == n
+        traj = trajectories(B, seed)
+        dx = api.DeviceArray.from_numpy(x)
+        dpos, dfr, dtraj = (api.DeviceArray.from_numpy(a) for a in (POS, FR, traj))
+        W = (n + S - 1) // S
+        dout = api.DeviceArray.empty((W, F, B), np.float32)
+        TR.multibeam_tracker_launch(None, dx, dout, n, F, A, B, S, dpos, dfr,
+                                    trajectories=dtraj, kernel=K.SPLIT_F16)
+        api.device_synchronize()
+        assert np.array_equal(dout.to_numpy(), got), f"request {i} vs multibeam tracker"
+        if B * n <= 64 * 1000:
+            dirs = np.zeros((W, B, 3), np.float32)
+            for w in range(W):
+                for b in range(B):
+                    dirs[w, b] = O.tracker_direction(*traj[b], w * S)
+            _, ref64 = O.multibeam_tracker(x, n, F, A, B, S, POS, FR, dirs, want_f64=True)
+            err = rel_err_floored(got, ref64, incoherent_floor(x, A, S))
+            assert err <= 1e-5, f"request {i}: {err:.3e}"
+    assert stats["windows_processed"] == sum((r[2] + r[1] - 1) // r[1] for r in MIX)
+    assert stats["windows_timed"] > 0 and stats["latency_p99_ms"] > 0
+
+
+def test_graph_and_eager_steps_agree(stream, mixed):
+    out, _ = mixed
+    eager, _ = run(MIX, stream, enable_graph=False)
+    for i in out:
+        assert np.array_equal(eager[i][1], out[i][1]), f"request {i}"
+
+
+def test_f16_kernel_holds_its_bar(stream):
+    B, S, n = 12, 320, 640
+    out, _ = run([(B, S, n, 0, 0, 10)], stream, kernel=K.F16)
+    x = np.ascontiguousarray(stream[:n])
+    traj = trajectories(B, 10)
+    W = n // S
+    dirs = np.array([[O.tracker_direction(*traj[b], w * S) for b in range(B)] for w in range(W)],
+                    np.float32)
+    _, ref64 = O.multibeam_tracker(x, n, F, A, B, S, POS, FR, dirs, want_f64=True)
+    assert rel_err_floored(out[0][1], ref64, incoherent_floor(x, A, S)) <= 2e-3
+
+
+def test_admission_waits_for_hbm_budget(stream):
+    B, S, n = 8, 10, 400
+    per_request = (n // S) * F * B * 4
+    srv = SV.TrackerServer(config(hbm_budget_bytes=2 * per_request + 256))
+    ids = [srv.submit(SV.Request(trajectories(B, 40 + i), S, n)) for i in range(3)]
+    srv.step()
+    st = [srv.status(r) for r in ids]
+    assert [s["state"] for s in st] == [SV.ACTIVE, SV.ACTIVE, SV.PENDING]
+    h = 0
+    while srv.status(ids[0])["state"] != SV.COMPLETE:
+        srv.ingest(stream[h:h + 100])
+        h += 100
+        srv.step()
+        srv.synchronize()
+    first = srv.read(ids[0], 0, n // S, B)
+    srv.release(ids[0])
+    srv.ingest(stream[h:h + 100])
+    h += 100
+    srv.step()
+    s2 = srv.status(ids[2])
+    # admitted once request 0's outputs were released; it starts at the head
+    assert s2["state"] == SV.ACTIVE and s2["start"] == h
+    while srv.status(ids[2])["state"] != SV.COMPLETE:
+        srv.ingest(stream[h:h + 100])
+        h += 100
+        srv.step()
+        srv.synchronize()
+    late = srv.read(ids[2], 0, n // S, B)
+    stats = srv.stats()
+    assert stats["requests_admitted"] == 3 and stats["arena_peak_bytes"] <= 2 * per_request + 256
+    assert stats["mean_admission_wait_spectra"] > 0
+    srv.close()
+    # the same two requests on a server with room for both
+    srv2 = SV.TrackerServer(config())
+    r0 = srv2.submit(SV.Request(trajectories(B, 40), S, n, 0))
+    r2 = srv2.submit(SV.Request(trajectories(B, 42), S, n, s2["start"]))
+    for g in range(0, h, 100):
+        srv2.ingest(stream[g:g + 100])
+        srv2.step()
+    srv2.synchronize()
+    assert np.array_equal(srv2.read(r0, 0, n // S, B), first)
+    assert np.array_equal(srv2.read(r2, 0, n // S, B), late)
+    srv2.close()
+
+
+def test_output_ring_and_reads(stream):
+    B, S, n = 3, 10, 500
+    srv = SV.TrackerServer(config())
+    full = srv.submit(SV.Request(trajectories(B, 7), S, n, 0))
+    ring = srv.submit(SV.Request(trajectories(B, 7), S, n, 0, output_windows=8))
+    for h in range(0, 500, 50):
+        srv.ingest(stream[h:h + 50])
+        srv.step()
+        done = srv.status(ring)["windows_issued"]
+        if done >= 4:
+            a = srv.read(ring, done - 4, 4, B)
+            b = srv.read(full, done - 4, 4, B)
+            assert np.array_equal(a, b)
+    with pytest.raises(_capi.OutOfRange):
+        srv.read(ring, 0, 4, B)  # overwritten
+    with pytest.raises(_capi.OutOfRange):
+        srv.read(full, 0, n // S + 1, B)  # not yet produced
+    srv.close()
+
+
+def test_ring_overrun_and_request_validation(stream):
+    srv = SV.TrackerServer(config(ring_spectra=1024))
+    srv.submit(SV.Request(trajectories(2, 1), 320, 5000, 0))
+    srv.ingest(stream[:600])
+    with pytest.raises(_capi.LogicError, match="ring overrun"):
+        srv.ingest(stream[600:1200])  # no step has processed the oldest spectra
+    srv.step()
+    srv.synchronize()
+    srv.ingest(stream[600:1200])  # fine once the oldest spectra are processed
+    with pytest.raises(_capi.InvalidArgument, match="n_beams"):
+        srv.submit(SV.Request(np.zeros((65, 4), np.float32), 10, 100))
+    with pytest.raises(_capi.InvalidArgument, match="window"):
+        srv.submit(SV.Request(np.zeros((1, 4), np.float32), 640, 1000))
+    with pytest.raises(_capi.InvalidArgument, match="sky"):
+        srv.submit(SV.Request(np.array([[0.5, 0.5, 1e-3, 0.0]], np.float32), 10, 1000))
+    old = srv.submit(SV.Request(trajectories(1, 2), 10, 100, start=0))
+    srv.step()
+    assert srv.status(old)["state"] == SV.REJECTED
+    srv.close()
+
+
+def test_native_load_generator_reports_latency():
+    cfg = SV.SimConfig(server=config(ring_spectra=1 << 16, max_requests=64,
+                                     hbm_budget_bytes=1 << 30),
+                       run_seconds=0.25, arrival_rate_hz=80.0, min_duration_s=0.02,
+                       max_duration_s=0.1, output_windows_cap=256, pool_spectra=4096)
+    rep = SV.simulate(cfg)
+    st = rep["stats"]
+    assert rep["requests_generated"] > 5
+    assert st["requests_admitted"] >= 1 and st["windows_timed"] > 0
+    assert 0 < st["latency_p50_ms"] <= st["latency_p99_ms"] <= st["latency_max_ms"]
+    assert st["beam_samples"] > 0 and rep["beam_samples_per_s"] > 0
+    assert rep["graph_used"] == 1
