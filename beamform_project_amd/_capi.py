@@ -58,6 +58,12 @@ class Shard(C.Structure):
                 ("loss_mask_independent", C.c_int)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("beams_per_wave", C.c_int), ("waves_per_workgroup", C.c_int),
+                ("double_buffered", C.c_int), ("resident_workgroups", C.c_int),
+                ("median_ms", C.c_double)]
+
+
 class FrameView(C.Structure):
     _fields_ = [("device_data", C.c_void_p), ("bytes", C.c_size_t), ("type", C.c_int),
                 ("rank", C.c_size_t), ("shape", C.c_size_t * 4),
@@ -80,6 +86,9 @@ SIGNATURES = {
     "bf_integrated_time_count": [_sz, _sz, C.POINTER(_sz)],
     "bf_quantization_parameter_count": [_pd, C.POINTER(_sz)],
     "bf_resolve_kernel": [C.c_int, _pd, _sz, C.POINTER(C.c_int)],
+    "bf_autotune_packed_beamformer": [C.c_int, _vp, _vp, _vp, _vp, _pd, _sz, C.c_int, _vp, _sz,
+                                      C.POINTER(_sz)],
+    "bf_autotune_clear": [],
     "bf_launch_packed_beamformer": [C.c_int, _vp, _vp, _vp, _vp, _pd],
     "bf_launch_packed_integrated_beamformer": [C.c_int, _vp, _vp, _vp, _vp, _pd, _sz],
     "bf_launch_quantize_integrated_intensity": [_vp, _vp, _vp, _vp, _pd],

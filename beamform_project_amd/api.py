@@ -231,6 +231,24 @@ def launch_quantize_integrated_intensity(stream, integrated_intensity, quantized
          _ptr(quantized_intensity), _ptr(parameters), C.byref(d))
 
 
+def autotune_packed_beamformer(kernel: Kernel, stream, packed_voltage, weights, output,
+                               dims: Dimensions, integration_spectra: int = 0,
+                               repeats: int = 3) -> list:
+    """Times every MFMA tile shape on these device buffers and keeps the
+    fastest for later launches of the same problem (bf_autotune_packed_beamformer).
+    Returns one dict per candidate."""
+    n = C.c_size_t()
+    entries = (_capi.TuneEntry * 64)()
+    call("bf_autotune_packed_beamformer", int(kernel), _sh(stream), _ptr(packed_voltage),
+         _ptr(weights), _ptr(output), C.byref(dims.c()), integration_spectra, repeats, entries,
+         64, C.byref(n))
+    return [{f: getattr(e, f) for f, _ in e._fields_} for e in entries[:min(n.value, 64)]]
+
+
+def autotune_clear() -> None:
+    call("bf_autotune_clear")
+
+
 def weight_storage_count(dims: Dimensions, kernel: Kernel) -> int:
     if kernel == Kernel.TILED:
         return dims.n_freq * ((dims.n_beams + 31) // 32) * dims.n_ant * 32

@@ -49,8 +49,11 @@ struct bf_doa {
     double* d_broadband = nullptr;
     double* d_peaks = nullptr;
     int* d_npeaks = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // around the 3 kernels
 
     ~bf_doa() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
         for (void* p : {(void*)d_freqs, (void*)d_theta, (void*)d_sin, (void*)d_rs, (void*)d_state,
                         (void*)d_capon, (void*)d_bart, (void*)d_ok, (void*)d_broadband,
                         (void*)d_peaks, (void*)d_npeaks})
@@ -112,6 +115,7 @@ bf_status bf_doa_create(const bf_doa_config* c, bf_doa** out) {
         d->d_broadband = dmalloc<double>(2 * n);
         d->d_peaks = dmalloc<double>(2 * c->max_peaks * 2);
         d->d_npeaks = dmalloc<int>(2);
+        for (hipEvent_t& e : d->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
         *out = d.release();
     });
 }
@@ -135,16 +139,36 @@ bf_status bf_doa_update(bf_doa* d, void* stream, const uint8_t* x, size_t n_time
         const bf_doa_config& c = d->cfg;
         if (t0 + c.snapshots > n_time) invalid("the update window exceeds n_time");
         hipStream_t s = static_cast<hipStream_t>(stream);
+        hip_check(hipEventRecord(d->ev[0], s), "hipEventRecord");
         launch_doa_covariance(s, x, n_time, c.n_freq, c.n_ant, t0, c.snapshots, d->M, d->d_rs,
                               nullptr);
+        hip_check(hipEventRecord(d->ev[1], s), "hipEventRecord");
         launch_doa_fold_scan(s, d->d_rs, d->d_state, !d->initialised, c.n_freq, d->M,
                              c.snapshots, d->L, c.forgetting, c.diagonal_load, d->d_freqs,
                              c.element_spacing_m, c.n_theta, d->d_sin, d->d_capon, d->d_bart,
                              d->d_ok);
+        hip_check(hipEventRecord(d->ev[2], s), "hipEventRecord");
         launch_doa_estimate(s, d->d_capon, d->d_bart, c.n_freq, c.n_theta, d->d_theta,
                             c.max_peaks, d->d_broadband, d->d_peaks, d->d_npeaks);
+        hip_check(hipEventRecord(d->ev[3], s), "hipEventRecord");
         d->initialised = true;
         d->last_stream = s;
+    });
+}
+
+bf_status bf_doa_last_timings(bf_doa* d, bf_doa_timings* t) {
+    return guarded([&] {
+        require_ptr(d, "doa");
+        require_ptr(t, "timings");
+        if (!d->initialised) fail(BF_ERR_LOGIC, "no update has run");
+        hip_check(hipEventSynchronize(d->ev[3]), "hipEventSynchronize");
+        float ms[3] = {0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < 3; ++i)
+            hip_check(hipEventElapsedTime(&ms[i], d->ev[i], d->ev[i + 1]),
+                      "hipEventElapsedTime");
+        t->covariance_ms = ms[0];
+        t->fold_scan_ms = ms[1];
+        t->estimate_ms = ms[2];
     });
 }
 

@@ -319,6 +319,40 @@ bf_status bf_launch_packed_integrated_beamformer(int kernel, void* stream,
     });
 }
 
+bf_status bf_autotune_packed_beamformer(int kernel, void* stream, const uint8_t* packed,
+                                        const float* weights, float* output,
+                                        const bf_dims* dims, size_t S, int repeats,
+                                        bf_tune_entry* entries, size_t max_entries,
+                                        size_t* n_entries) {
+    return guarded([&] {
+        validate_kernel(kernel);
+        const bf_dims& d = deref_dims(dims);
+        validate_dimensions(d);
+        if (S != 0) validate_integration(kernel, S);
+        const int k = resolve(kernel, d, S);
+        if (k != BF_KERNEL_SPLIT_F16 && k != BF_KERNEL_F16)
+            invalid("the autotuner covers the SplitF16 and F16 MFMA kernels");
+        require_ptr(packed, "packed_voltage");
+        require_ptr(weights, "weights");
+        require_ptr(output, "output");
+        require_aligned(packed, 16, "packed_voltage");
+        require_aligned(weights, 8, "weights");
+        require_aligned(output, 4, "output");
+        const auto res = autotune_mfma_fused(static_cast<hipStream_t>(stream),
+                                             k == BF_KERNEL_SPLIT_F16, packed, weights, output,
+                                             d.n_time, d.n_freq, d.n_ant, d.n_beams, S,
+                                             repeats);
+        if (n_entries) *n_entries = res.size();
+        for (size_t i = 0; entries && i < res.size() && i < max_entries; ++i)
+            entries[i] = bf_tune_entry{res[i].cfg.nt * 8, res[i].cfg.waves, res[i].cfg.db,
+                                       res[i].cfg.resident, (double)res[i].ms};
+    });
+}
+
+bf_status bf_autotune_clear(void) {
+    return guarded([&] { clear_mfma_tuning(); });
+}
+
 bf_status bf_launch_quantize_integrated_intensity(void* stream, const float* integrated,
                                                   int8_t* codes, float* params,
                                                   const bf_dims* integrated_dims) {

@@ -1,380 +1,68 @@
-// k_mfma.hip -- FP16-MFMA fused fixed-grid beamformer for gfx950.
+// k_mfma.hip -- dispatch and autotuning of the MFMA fixed-grid beamformer.
 //
-// Per frequency f the contraction y[t][b] = sum_a W[b][f][a] * X[t][f][a]
-// (reference src/cpu_beamformer.cpp:56-78, no conjugate) is a real GEMM
-//   [xr0 xi0 xr1 xi1 ...] (1 x 2A)  *  B (2A x 2B),
-// where output column 2b holds Re(y_b) (weights (wr, -wi) per antenna) and
-// column 2b+1 holds Im(y_b) (weights (wi, wr)).  int4 samples are exact in
-// FP16, so the only rounding in the products comes from the weights: the
-// SPLIT variant stores each weight as hi = fp16(w) plus lo = fp16(w - hi)
-// and runs the K loop twice over the same A operand (products are exact in
-// the FP32 accumulator), recovering ~22 significant bits per weight.
-//
-// Instruction: v_mfma_f32_16x16x32_f16.  A workgroup of 4 waves owns one
-// frequency, one chunk of time and 4*NT*8 beams; each wave keeps the
-// weights of NT N-tiles (8 beams each, re/im interleaved) in VGPRs for the
-// whole chunk.  The packed int4 rows of a stage (ST tiles of 16 spectra)
-// are loaded once per workgroup with coalesced dword loads, converted to
-// FP16 pairs (byte permute, xor, one packed subtraction) and written to LDS
-// in the MFMA A-operand layout, so each lane reads its operand with one
-// 16-byte LDS load per k-step.  The next stage's loads are issued before the
-// current stage is computed.
-//
-// Output layout of a 16x16 MFMA: lane l holds column l%16 for rows
-// 4*(l/16)+r, r = 0..3.  Tile rows are mapped to time so that row group
-// G = l/16 of a super-tile covers consecutive spectra: each lane then
-// accumulates whole integration windows in registers.
-//   MODE 0 (no integration): tile i, row 4G+r -> t = 16i + 4G + r
-//   MODE 1 (S == 10): super-tile of 5 tiles, G covers 20 spectra (2 windows)
-//   MODE 2 (S % 4 == 0): super-tile of S/4 tiles, G covers one window
-// Re and im of a beam sit in adjacent lanes; a quad-permute DPP adds the
-// two squared sums.
-#include "bf_kernels.hpp"
+// The kernel template lives in mfma_fused.hpp and is instantiated per
+// antenna count in k_mfma_a*.hip for a small set of tile shapes (beams per
+// wave, waves per workgroup, single or double LDS stage).  A launch takes
+// the shape the autotuner measured best for its exact problem, or the
+// default below.  The autotuner times every instantiated shape (and three
+// chunk-sizing assumptions) on the caller's buffers with device events.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "mfma_fused.hpp"
 
 namespace bfhip {
 namespace {
 
-constexpr int kWaveSize = 64;
-constexpr int kWaves = 4;
-constexpr int kThreads = kWaves * kWaveSize;
-constexpr int kXcds = 8;
+using TuneKey = std::tuple<int, int, int, int, int, bool>;  // T, F, A, B, S, split
 
-__device__ __forceinline__ float dpp_swap_pairs(float v) {
-    // quad_perm [1,0,3,2]: lane i <- lane i^1.
-    return __builtin_bit_cast(
-        float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true));
-}
-
-// Time of row (G, r) of global tile i.
-template <int MODE>
-__device__ __forceinline__ int tile_time(int i, int G, int r, int S) {
-    if (MODE == 0) return 16 * i + 4 * G + r;
-    if (MODE == 1) return 80 * (i / 5) + 20 * G + 4 * (i % 5) + r;
-    const int tps = S >> 2;
-    return 4 * S * (i / tps) + S * G + 4 * (i % tps) + r;
-}
-
-// Loads the B operand (weights) of one wave into registers.
-template <int A, int NT, bool SPLIT>
-__device__ __forceinline__ void load_weights(const float2* __restrict__ w, int F, int B,
-                                             int f, int beam0, int lane,
-                                             bf_half8 (&bhi)[NT][A / 16],
-                                             bf_half8 (&blo)[NT][A / 16]) {
-    constexpr int KS = A / 16;
-    const int c = lane & 15;
-    const int g = lane >> 4;
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-        const int b = beam0 + nt * 8 + (c >> 1);
-        const bool comp_im = (c & 1) != 0;
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            bf_half8 h, l;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int a = g * (A / 4) + 4 * s + j;
-                float2 v = make_float2(0.0f, 0.0f);
-                if (b < B) v = w[((size_t)b * F + f) * A + a];
-                // column re: (wr, -wi); column im: (wi, wr)
-                const float e0 = comp_im ? v.y : v.x;
-                const float e1 = comp_im ? v.x : -v.y;
-                const _Float16 h0 = (_Float16)e0;
-                const _Float16 h1 = (_Float16)e1;
-                h[2 * j] = h0;
-                h[2 * j + 1] = h1;
-                if (SPLIT) {
-                    l[2 * j] = (_Float16)(e0 - (float)h0);
-                    l[2 * j + 1] = (_Float16)(e1 - (float)h1);
-                }
-            }
-            bhi[nt][s] = h;
-            if (SPLIT) blo[nt][s] = l;
-        }
-    }
-}
-
-template <int A, int NT, bool SPLIT, int MODE, int ST>
-__global__ __launch_bounds__(kThreads) void mfma_fused_kernel(
-    const uint8_t* __restrict__ x, const float2* __restrict__ w, float* __restrict__ out,
-    int T, int F, int B, int S, int W, int tiles_total, int tiles_per_chunk, int n_items,
-    int per_xcd) {
-    constexpr int KS = A / 16;                     // k-steps of 32 (16 antennas x re/im)
-    constexpr int TILE_BYTES = KS * kWaveSize * 16; // LDS bytes per 16-row tile
-    constexpr int DW_ROW = A / 4;                   // packed dwords per row
-    constexpr int DW_TILE = 16 * DW_ROW;
-    constexpr int DW_STAGE = ST * DW_TILE;
-    constexpr int DW_THREAD = (DW_STAGE + kThreads - 1) / kThreads;
-    __shared__ __attribute__((aligned(16))) unsigned char lds[ST * TILE_BYTES];
-
-    const int tid = threadIdx.x;
-    const int lane = tid & (kWaveSize - 1);
-    const int wave = tid / kWaveSize;
-    // XCD-aware order: workgroups are dispatched round-robin over the 8
-    // XCDs, so blockIdx.x % 8 names the XCD.  Each XCD takes a contiguous
-    // range of frequency-major items: all time chunks of a channel run on
-    // one XCD back to back (its 64 KB of weights stay in that L2) and the
-    // neighbouring channel, which shares the 128-byte lines of every packed
-    // row, follows on the same XCD.
-    const int item = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
-    if (item >= n_items) return;  // whole workgroup
-    const int n_chunks = n_items / F;
-    const int f = item / n_chunks;
-    const int chunk = item - f * n_chunks;
-    const int beam0 = (blockIdx.y * kWaves + wave) * NT * 8;
-    const bool has_beams = beam0 < B;
-
-    bf_half8 bhi[NT][KS];
-    bf_half8 blo[NT][KS];
-    if (has_beams) load_weights<A, NT, SPLIT>(w, F, B, f, beam0, lane, bhi, blo);
-
-    const int tile_begin = chunk * tiles_per_chunk;
-    int tile_end = tile_begin + tiles_per_chunk;
-    tile_end = tile_end < tiles_total ? tile_end : tiles_total;
-    const size_t row_stride = (size_t)F * A;
-    const uint8_t* xf = x + (size_t)f * A;
-
-    // Stage loader: dword q of the stage -> (tile, row, dword in row).
-    unsigned pre[DW_THREAD];
-    auto load_stage = [&](int stage_tile0) {
-#pragma unroll
-        for (int k = 0; k < DW_THREAD; ++k) {
-            const int q = tid + k * kThreads;
-            unsigned v = 0u;
-            if (q < DW_STAGE) {
-                const int tl = q / DW_TILE;
-                const int rem = q - tl * DW_TILE;
-                const int rho = rem / DW_ROW;
-                const int j = rem - rho * DW_ROW;
-                const int t = tile_time<MODE>(stage_tile0 + tl, rho >> 2, rho & 3, S);
-                if (t < T)
-                    v = *reinterpret_cast<const unsigned*>(xf + (size_t)t * row_stride + 4 * j);
-            }
-            pre[k] = v;
-        }
-    };
-    auto store_stage = [&]() {
-#pragma unroll
-        for (int k = 0; k < DW_THREAD; ++k) {
-            const int q = tid + k * kThreads;
-            if (q < DW_STAGE) {
-                const int tl = q / DW_TILE;
-                const int rem = q - tl * DW_TILE;
-                const int rho = rem / DW_ROW;
-                const int j = rem - rho * DW_ROW;
-                const int g = j / (A / 16);
-                const int s = j - g * (A / 16);
-                unsigned h2[4];
-                bf_int4x4_to_half2x4(pre[k], h2);
-                bf_uint4v u = {h2[0], h2[1], h2[2], h2[3]};
-                *reinterpret_cast<bf_uint4v*>(lds + tl * TILE_BYTES +
-                                              (s * kWaveSize + rho + 16 * g) * 16) = u;
-            }
-        }
-    };
-
-    float win[NT][2];
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) win[nt][0] = win[nt][1] = 0.0f;
-    const int col = lane & 15;
-    const int G = lane >> 4;
-    const int tps = MODE == 2 ? (S >> 2) : 5;
-
-    if (tile_begin < tile_end) load_stage(tile_begin);
-    for (int st0 = tile_begin; st0 < tile_end; st0 += ST) {
-        __syncthreads();  // previous stage fully consumed
-        store_stage();
-        __syncthreads();  // stage visible to all waves
-        if (st0 + ST < tile_end) load_stage(st0 + ST);  // in flight during compute
-        if (!has_beams) continue;
-        // One tile ahead: the A operand of tile tl+1 is read from LDS while
-        // the MFMAs of tile tl run.
-        bf_half8 av[KS];
-#pragma unroll
-        for (int s = 0; s < KS; ++s)
-            av[s] = *reinterpret_cast<const bf_half8*>(lds + (s * kWaveSize + lane) * 16);
-#pragma unroll 1
-        for (int tl = 0; tl < ST; ++tl) {
-            const int i = st0 + tl;
-            bf_half8 an[KS];
-            if (tl + 1 < ST) {
-#pragma unroll
-                for (int s = 0; s < KS; ++s)
-                    an[s] = *reinterpret_cast<const bf_half8*>(
-                        lds + (tl + 1) * TILE_BYTES + (s * kWaveSize + lane) * 16);
-            }
-            bf_float4 acc[NT];
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[nt] = bf_float4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int s = 0; s < KS; ++s) {
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[s], bhi[nt][s], acc[nt],
-                                                                     0, 0, 0);
-                if (SPLIT) {
-#pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
-                        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[s], blo[nt][s],
-                                                                         acc[nt], 0, 0, 0);
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < KS; ++s) av[s] = an[s];
-            if (MODE == 0) {
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    const int b = beam0 + nt * 8 + (col >> 1);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const float sq = acc[nt][r] * acc[nt][r];
-                        const float p = sq + dpp_swap_pairs(sq);
-                        const int t = tile_time<0>(i, G, r, S);
-                        if ((col & 1) == 0 && b < B && t < T)
-                            out[((size_t)t * F + f) * B + b] = p;
-                    }
-                }
-                continue;
-            }
-            const int ii = MODE == 1 ? tl : (i % tps);  // tile within super-tile
-            if (MODE == 1 && tl == 2) {
-                // spectra 8, 9 close window 0; 10, 11 open window 1
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    win[nt][0] = fmaf(acc[nt][0], acc[nt][0], win[nt][0]);
-                    win[nt][0] = fmaf(acc[nt][1], acc[nt][1], win[nt][0]);
-                    win[nt][1] = fmaf(acc[nt][2], acc[nt][2], win[nt][1]);
-                    win[nt][1] = fmaf(acc[nt][3], acc[nt][3], win[nt][1]);
-                }
-            } else {
-                const int q = (MODE == 1 && tl > 2) ? 1 : 0;  // wave-uniform
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    float a = 0.0f;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) a = fmaf(acc[nt][r], acc[nt][r], a);
-                    if (q) win[nt][1] += a;
-                    else win[nt][0] += a;
-                }
-            }
-            if (ii == tps - 1) {
-                constexpr int kWinPerGroup = MODE == 1 ? 2 : 1;
-                const int super = i / tps;
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    const int b = beam0 + nt * 8 + (col >> 1);
-#pragma unroll
-                    for (int q = 0; q < kWinPerGroup; ++q) {
-                        const float p = win[nt][q] + dpp_swap_pairs(win[nt][q]);
-                        const int wi = super * 4 * kWinPerGroup + G * kWinPerGroup + q;
-                        if ((col & 1) == 0 && b < B && wi < W)
-                            out[((size_t)wi * F + f) * B + b] = p;
-                        win[nt][q] = 0.0f;
-                    }
-                }
-            }
-        }
-    }
-}
-
-struct Plan {
-    int mode, st, tiles_total, tiles_per_chunk, n_items, per_xcd, beam_blocks, W;
-};
-
-Plan make_plan(int T, int F, int B, int S, int NT) {
-    Plan p{};
-    p.mode = S == 0 ? 0 : (S == 10 ? 1 : 2);
-    int tiles_per_super = 1;
-    if (p.mode == 0) {
-        p.st = 4;
-        p.W = T;
-    } else if (p.mode == 1) {
-        p.st = 5;
-        tiles_per_super = 5;
-        p.W = (int)ceil_div((size_t)T, 10);
-    } else {
-        tiles_per_super = S / 4;
-        p.st = tiles_per_super % 4 == 0 ? 4 : 1;
-        p.W = (int)ceil_div((size_t)T, (size_t)S);
-    }
-    const int rows_super = 16 * tiles_per_super;
-    const int n_super = (int)ceil_div((size_t)T, (size_t)rows_super);
-    p.tiles_total = n_super * tiles_per_super;
-    if (p.mode == 0) p.tiles_total = (int)ceil_div((size_t)p.tiles_total, 4) * 4;
-    // Chunk length: a whole number of super-tiles and stages, chosen to
-    // minimise (rounds of resident workgroups) x (tiles per workgroup +
-    // a fixed start-up cost for loading the weights), with 512 workgroups
-    // resident (256 CUs x 2).
-    int unit = tiles_per_super;
-    while (unit % p.st) unit += tiles_per_super;
-    const long resident = 512, startup_tiles = 12;
-    long best_cost = -1;
-    int per_chunk = unit;
-    for (int c = unit; c <= 640 && c <= p.tiles_total + unit; c += unit) {
-        const long chunks = (long)ceil_div((size_t)p.tiles_total, (size_t)c);
-        const long rounds = (long)ceil_div((size_t)(chunks * F), (size_t)resident);
-        const long cost = rounds * (c + startup_tiles);
-        if (best_cost < 0 || cost < best_cost) {
-            best_cost = cost;
-            per_chunk = c;
-        }
-    }
-    p.tiles_per_chunk = per_chunk;
-    const int n_chunks = (int)ceil_div((size_t)p.tiles_total, (size_t)per_chunk);
-    p.n_items = n_chunks * F;
-    p.per_xcd = (int)ceil_div((size_t)p.n_items, kXcds);
-    p.beam_blocks = (int)ceil_div((size_t)B, (size_t)(kWaves * NT * 8));
-    return p;
-}
-
-template <int A, int NT, bool SPLIT>
-void launch_nt(hipStream_t stream, const uint8_t* x, const float* w, float* out, int T, int F,
-               int B, int S) {
-    const Plan p = make_plan(T, F, B, S, NT);
-    const dim3 grid((unsigned)(p.per_xcd * kXcds), (unsigned)p.beam_blocks);
-    const dim3 block(kThreads);
-    const float2* w2 = reinterpret_cast<const float2*>(w);
-#define BF_LAUNCH(MODE_, ST_)                                                                 \
-    hipLaunchKernelGGL((mfma_fused_kernel<A, NT, SPLIT, MODE_, ST_>), grid, block, 0, stream, \
-                       x, w2, out, T, F, B, S, p.W, p.tiles_total, p.tiles_per_chunk,          \
-                       p.n_items, p.per_xcd)
-    if (p.mode == 0) {
-        BF_LAUNCH(0, 4);
-    } else if (p.mode == 1) {
-        BF_LAUNCH(1, 5);
-    } else if (p.st == 4) {
-        BF_LAUNCH(2, 4);
-    } else {
-        BF_LAUNCH(2, 1);
-    }
-#undef BF_LAUNCH
+std::mutex g_tune_mu;
+std::map<TuneKey, MfmaConfig>& tune_cache() {
+    static std::map<TuneKey, MfmaConfig> cache;
+    return cache;
 }
 
 // Beams per wave: as many 8-beam N-tiles as the beam count needs, capped so
 // that the split weights take at most 128 VGPRs per lane.
-template <int A, bool SPLIT>
-void launch_a(hipStream_t stream, const uint8_t* x, const float* w, float* out, int T, int F,
-              int B, int S) {
-    constexpr int kMaxNT = A == 32 ? 4 : (A == 64 ? 4 : (A == 128 ? 2 : 1));
-    const int need = (int)ceil_div((size_t)B, (size_t)(kWaves * 8));
-    if (need <= 1 || kMaxNT == 1) {
-        launch_nt<A, 1, SPLIT>(stream, x, w, out, T, F, B, S);
-    } else if (need <= 2 || kMaxNT == 2) {
-        launch_nt<A, (kMaxNT >= 2 ? 2 : 1), SPLIT>(stream, x, w, out, T, F, B, S);
-    } else {
-        launch_nt<A, kMaxNT, SPLIT>(stream, x, w, out, T, F, B, S);
+MfmaConfig default_config(int A, int B) {
+    const int max_nt = A <= 64 ? 4 : (A == 128 ? 2 : 1);
+    const int need = (B + 31) / 32;
+    MfmaConfig c;
+    c.waves = 4;
+    c.nt = need <= 1 ? 1 : (need <= 2 ? std::min(2, max_nt) : max_nt);
+    if (A == 256) c.nt = 1;
+    c.db = 0;
+    c.resident = 512;
+    return c;
+}
+
+bool launch_cfg(hipStream_t stream, const MfmaLaunch& L, int A, const MfmaConfig& cfg) {
+    switch (A) {
+    case 32: return launch_mfma_a32(stream, L, cfg);
+    case 64: return launch_mfma_a64(stream, L, cfg);
+    case 128: return launch_mfma_a128(stream, L, cfg);
+    case 256: return launch_mfma_a256(stream, L, cfg);
+    default: invalid("n_ant must be 32, 64, 128, or 256");
     }
 }
 
-template <bool SPLIT>
-void launch_split(hipStream_t stream, const uint8_t* x, const float* w, float* out, int T,
-                  int F, int A, int B, int S) {
-    switch (A) {
-    case 32: launch_a<32, SPLIT>(stream, x, w, out, T, F, B, S); break;
-    case 64: launch_a<64, SPLIT>(stream, x, w, out, T, F, B, S); break;
-    case 128: launch_a<128, SPLIT>(stream, x, w, out, T, F, B, S); break;
-    case 256: launch_a<256, SPLIT>(stream, x, w, out, T, F, B, S); break;
-    default: invalid("n_ant must be 32, 64, 128, or 256");
-    }
+std::vector<std::pair<int, int>> shapes_for(int A) {
+    if (A <= 64) return {{4, 4}, {2, 8}, {2, 4}, {1, 8}};
+    if (A == 128) return {{2, 4}, {1, 8}, {1, 4}};
+    return {{1, 4}};
+}
+
+void check_problem(size_t T, size_t F, size_t A, size_t B, size_t S) {
+    if (!mfma_supported(A, B, S))
+        invalid("the MFMA beamformer supports integration of 10 spectra or a multiple of 4");
+    if (T > (size_t)INT32_MAX / 8 || S > (size_t)INT32_MAX / 8 || F > 65535 ||
+        T * F > ((size_t)1 << 40))
+        fail(BF_ERR_OVERFLOW, "MFMA beamformer grid exceeds the supported size");
 }
 
 }  // namespace
@@ -388,15 +76,81 @@ bool mfma_supported(size_t A, size_t B, size_t S) {
 
 void launch_mfma_fused(hipStream_t stream, bool split, const uint8_t* x, const float* w,
                        float* out, size_t T, size_t F, size_t A, size_t B, size_t S) {
-    if (!mfma_supported(A, B, S))
-        invalid("the MFMA beamformer supports integration of 10 spectra or a multiple of 4");
-    if (T > (size_t)INT32_MAX / 8 || S > (size_t)INT32_MAX / 8 || T * F > ((size_t)1 << 40))
-        fail(BF_ERR_OVERFLOW, "MFMA beamformer grid exceeds the supported size");
-    if (split)
-        launch_split<true>(stream, x, w, out, (int)T, (int)F, (int)A, (int)B, (int)S);
-    else
-        launch_split<false>(stream, x, w, out, (int)T, (int)F, (int)A, (int)B, (int)S);
+    check_problem(T, F, A, B, S);
+    MfmaConfig cfg = default_config((int)A, (int)B);
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = tune_cache().find(TuneKey((int)T, (int)F, (int)A, (int)B, (int)S, split));
+        if (it != tune_cache().end()) cfg = it->second;
+    }
+    const MfmaLaunch L{x, w, out, (int)T, (int)F, (int)B, (int)S, split};
+    if (!launch_cfg(stream, L, (int)A, cfg)) invalid("MFMA tile shape is not instantiated");
     hip_check(hipGetLastError(), "mfma_fused_kernel launch");
+}
+
+std::vector<MfmaTuneResult> autotune_mfma_fused(hipStream_t stream, bool split,
+                                                const uint8_t* x, const float* w, float* out,
+                                                size_t T, size_t F, size_t A, size_t B,
+                                                size_t S, int repeats) {
+    check_problem(T, F, A, B, S);
+    if (repeats < 1) invalid("repeats must be positive");
+    const MfmaLaunch L{x, w, out, (int)T, (int)F, (int)B, (int)S, split};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    std::vector<MfmaTuneResult> results;
+    try {
+        for (const auto& sh : shapes_for((int)A)) {
+            for (int db = 0; db < 2; ++db) {
+                for (int resident : {256, 512, 1024}) {
+                    MfmaConfig cfg;
+                    cfg.nt = sh.first;
+                    cfg.waves = sh.second;
+                    cfg.db = db;
+                    cfg.resident = resident;
+                    if (!launch_cfg(stream, L, (int)A, cfg)) continue;  // warm-up
+                    hip_check(hipGetLastError(), "mfma_fused_kernel launch");
+                    std::vector<float> ms;
+                    for (int r = 0; r < repeats; ++r) {
+                        hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+                        launch_cfg(stream, L, (int)A, cfg);
+                        hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+                        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                        float t = 0.0f;
+                        hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+                        ms.push_back(t);
+                    }
+                    std::sort(ms.begin(), ms.end());
+                    results.push_back(MfmaTuneResult{cfg, ms[ms.size() / 2]});
+                }
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const auto best = std::min_element(
+        results.begin(), results.end(),
+        [](const MfmaTuneResult& a, const MfmaTuneResult& b) { return a.ms < b.ms; });
+    if (best != results.end()) {
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mu);
+            tune_cache()[TuneKey((int)T, (int)F, (int)A, (int)B, (int)S, split)] = best->cfg;
+        }
+        // the output ends with the winner's result (every shape computes the same sums)
+        launch_cfg(stream, L, (int)A, best->cfg);
+        hip_check(hipGetLastError(), "mfma_fused_kernel launch");
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+    return results;
+}
+
+void clear_mfma_tuning() {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    tune_cache().clear();
 }
 
 }  // namespace bfhip

@@ -19,6 +19,11 @@ class DoaConfigC(C.Structure):
                 ("element_spacing_m", C.c_double), ("frequencies", C.c_void_p)]
 
 
+class DoaTimingsC(C.Structure):
+    _fields_ = [("covariance_ms", C.c_double), ("fold_scan_ms", C.c_double),
+                ("estimate_ms", C.c_double)]
+
+
 _vp, _sz = C.c_void_p, C.c_size_t
 for _name, _args in {
     "bf_doa_create": [C.POINTER(DoaConfigC), C.POINTER(_vp)],
@@ -30,6 +35,7 @@ for _name, _args in {
     "bf_doa_read_covariance": [_vp, _vp],
     "bf_doa_get_state": [_vp, _vp, C.POINTER(C.c_int)],
     "bf_doa_set_state": [_vp, _vp, C.c_int],
+    "bf_doa_last_timings": [_vp, C.POINTER(DoaTimingsC)],
 }.items():
     _capi.register(_name, _args)
 
@@ -91,6 +97,11 @@ class DoaEstimator:
         call("bf_doa_read_results", self._h, bb.ctypes.data, pk.ctypes.data, npk.ctypes.data)
         return {"capon": bb[0], "bartlett": bb[1],
                 "capon_peaks": pk[0, :npk[0]], "bartlett_peaks": pk[1, :npk[1]]}
+
+    def last_timings(self) -> dict:
+        t = DoaTimingsC()
+        call("bf_doa_last_timings", self._h, C.byref(t))
+        return {n: getattr(t, n) for n, _ in t._fields_}
 
     def channel_spectra(self):
         n = self.cfg.n_theta

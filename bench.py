@@ -42,7 +42,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "b1"])
+    p.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "b1"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-serving", action="store_true")
@@ -99,8 +99,9 @@ def main():
         sys.exit(2)
     from beamform_project_amd import api
 
-    if args.workload in ("c3", "b1", "c4"):
-        line = {"c3": bench_c3, "b1": bench_b1, "c4": bench_c4}[args.workload](args)
+    if args.workload in ("c3", "b1", "c4", "c5"):
+        line = {"c3": bench_c3, "b1": bench_b1, "c4": bench_c4,
+                "c5": bench_c5}[args.workload](args)
         print(json.dumps(line), flush=True)
         return
     wl = workload(args.workload)
@@ -417,6 +418,85 @@ def bench_c4(args):
         args, "beam-samples/s (T·F·B); p99 per-window latency", sv["beam_samples_per_s"],
         "beam-samples/s", sv["wall_seconds"] * 1e3, "f16 split hi+lo, f32 accumulate",
         {"workload": sv["workload"]}, p99_window_latency_ms=sv["latency_p99_ms"], serving=sv)
+
+
+def bench_c5(args):
+    """Adaptive DOA: one update = K=1024 snapshots x 336 channels through
+    the int8-MFMA covariance, FP64 fold/Cholesky/scans and the estimates."""
+    from beamform_project_amd import api, doa as D
+    A, F_, K, M, NTH = 64, 336, 1024, 48, 721
+    n_upd = 8
+    T = K * n_upd
+    pos = np.zeros((A, 3), np.float32)
+    pos[:, 0] = 0.3 * np.arange(A)
+    fr = np.linspace(400e6, 800e6, F_).astype(np.float32)
+    th0 = np.deg2rad(17.3)
+    stream = api.Stream()
+    x = api.DeviceArray.empty((T, F_, A), np.uint8)
+    dpos, dfr = api.DeviceArray.from_numpy(pos), api.DeviceArray.from_numpy(fr)
+    sigma = 2.5
+    api.generate_point_source_noise(stream, x, T, F_, A, dpos, dfr,
+                                    np.array([np.sin(th0), 0.0, np.cos(th0)], np.float32),
+                                    float(np.sqrt(3.0 * 2 * sigma * sigma)), sigma, 505)
+    est = D.DoaEstimator(D.DoaConfig(fr, n_ant=A, snapshots=K, subarray=M, n_theta=NTH,
+                                     forgetting=0.98, diagonal_load=1e-3,
+                                     element_spacing_m=0.3))
+    for i in range(max(args.warmup, 1)):
+        est.update(x, T, (i % n_upd) * K, stream)
+    stream.synchronize()
+    e0, e1 = api.Event(), api.Event()
+    wall, kern = [], {"covariance_ms": [], "fold_scan_ms": [], "estimate_ms": []}
+    for i in range(args.steps):
+        e0.record(stream)
+        est.update(x, T, (i % n_upd) * K, stream)
+        e1.record(stream)
+        wall.append(e0.elapsed_ms(e1))
+        for k_, v in est.last_timings().items():
+            kern[k_].append(v)
+    ms = float(np.mean(wall))
+    kt = {k_: round(float(np.mean(v)), 4) for k_, v in kern.items()}
+    peaks = np.rad2deg(est.results()["capon_peaks"][:, 0]).tolist()
+    cov_bytes = K * F_ * A + F_ * M * M * 2 * 4
+    ach = cov_bytes / (kt["covariance_ms"] * 1e-3) / 1e9
+    cpu = None if args.no_cpu_baseline else doa_cpu_baseline(args.cpu_seconds, A, K, M, NTH)
+    return common_line(
+        args, "DOA snapshot-channels/s (K·F per update)", K * F_ / (ms * 1e-3),
+        "snapshot-channels/s", ms, "int8 MFMA covariance (int32 exact), f64 fold/Cholesky/scan",
+        {"workload": "C5 adaptive DOA: A=64 ULA 0.3 m, F=336 (400-800 MHz), K=1024 snapshots "
+                     "per update, forgetting 0.98, forward smoothing M=48 (L=17), 721 angles "
+                     "over +-90 deg, diagonal load 1e-3 tr(R)/M, one source at 17.3 deg + noise",
+         "n_ant": A, "n_freq": F_, "snapshots": K, "subarray": M, "n_theta": NTH},
+        updates_per_s=round(1e3 / ms, 1), kernel_ms=kt, capon_peaks_deg=peaks,
+        roofline={"bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
+                  "frac": round(ach / 8000.0, 4), "traffic": None,
+                  "kernel": "covariance", "kernel_avg_ms": kt["covariance_ms"],
+                  "algorithmic_bytes": cov_bytes},
+        cpu_baseline=cpu)
+
+
+def doa_cpu_baseline(seconds, A, K, M, NTH):
+    """The FP64 C restatement of the DOA update (oracle, single thread) on a
+    bounded subset of channels, scaled to snapshot-channels/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker/baseline library, never the product path
+
+    def run(Fs):
+        x = O.make_uniform_int4(K * Fs * A, 3).reshape(K, Fs, A)
+        fr = np.linspace(400e6, 800e6, Fs).astype(np.float32)
+        xpos = (0.3 * np.arange(A)).astype(np.float32)
+        theta = np.linspace(-np.pi / 2, np.pi / 2, NTH)
+        state = np.zeros((Fs, M, M, 2), np.float64)
+        t0 = time.perf_counter()
+        O.doa_update(x, K, Fs, A, 0, K, M, 0.98, 1e-3, True, fr, xpos, theta, state)
+        return time.perf_counter() - t0
+
+    dt = run(1)
+    Fs = int(max(1, min(336, seconds / 3 / max(dt, 1e-6))))
+    med = float(np.median([run(Fs) for _ in range(3)]))
+    return {"value": round(K * Fs / med, 1), "unit": "snapshot-channels/s", "cores": 1,
+            "kind": "port",
+            "sample": f"one update over {Fs} channels (K={K}, A={A}, M={M}, {NTH} angles) of "
+                      f"the FP64 C restatement; median of 3 runs of {med:.2f} s"}
 
 
 def fp64_error(x, out, w, A, B, S, n_windows=32):

@@ -123,6 +123,32 @@ bf_status bf_launch_quantize_integrated_intensity(void* stream,
 bf_status bf_resolve_kernel(int kernel, const bf_dims* dims, size_t integration_spectra,
                             int* resolved);
 
+/* Autotuner of the MFMA kernels (addition of this build; the reference
+ * picks its CUDA block shapes statically).  Times every instantiated tile
+ * shape -- beams per wave, waves per workgroup, single or double LDS stage,
+ * chunk sizing -- on the caller's device buffers (the layouts of
+ * bf_launch_packed_integrated_beamformer; integration_spectra 0 = the
+ * per-spectrum launcher), `repeats` timed launches each, and makes the
+ * fastest the shape of every later launch of the same problem (T, F, A, B,
+ * S, kernel) in this process.  Synchronous; allocates nothing on the
+ * device; leaves a valid result in `output` (every shape computes the same
+ * sums).  entries may be NULL; n_entries receives the candidate count. */
+typedef struct bf_tune_entry {
+    int beams_per_wave;
+    int waves_per_workgroup;
+    int double_buffered;
+    int resident_workgroups;
+    double median_ms;
+} bf_tune_entry;
+
+bf_status bf_autotune_packed_beamformer(int kernel, void* stream, const uint8_t* packed_voltage,
+                                        const float* weights, float* output,
+                                        const bf_dims* dims, size_t integration_spectra,
+                                        int repeats, bf_tune_entry* entries,
+                                        size_t max_entries, size_t* n_entries);
+/* Forgets every tuned shape (later launches use the defaults). */
+bf_status bf_autotune_clear(void);
+
 /* ------------------------------------------------------- timings */
 /* Reference: include/beamformer/cuda_beamformer.hpp:22-31. */
 typedef struct bf_timings {
@@ -384,6 +410,14 @@ bf_status bf_doa_read_channel_spectra(bf_doa* doa, double* capon, double* bartle
                                       int* chol_ok);
 /* Exact smoothed covariance of the last update, [F][M][M][2] int32. */
 bf_status bf_doa_read_covariance(bf_doa* doa, int32_t* rs);
+/* Device time of the last update's kernels (waits for it).  Reference
+ * analogue: CudaBeamformerTimings (cuda_beamformer.hpp:22-31). */
+typedef struct bf_doa_timings {
+    double covariance_ms;
+    double fold_scan_ms;
+    double estimate_ms;
+} bf_doa_timings;
+bf_status bf_doa_last_timings(bf_doa* doa, bf_doa_timings* timings);
 /* Checkpoint / resume of the recursion: R [F][M][M][2] FP64 and whether
  * an update has been folded. */
 bf_status bf_doa_get_state(bf_doa* doa, double* state, int* initialised);

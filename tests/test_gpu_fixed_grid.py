@@ -333,3 +333,32 @@ def test_full_shard_sampled_frequencies():
     ref = O.integrate_f64(i64, T, len(sample), B, S)
     err = rel_err_floored(got[:, sample], ref, incoherent_floor(x[:, sample], A, S))
     assert err <= SPLIT_TOL, err
+
+
+@pytest.mark.parametrize("T,A,B,S,kernel", [
+    (1600, 64, 128, 10, K.SPLIT_F16),
+    (640, 64, 40, 0, K.F16),
+    (960, 128, 33, 320, K.SPLIT_F16),
+    (256, 256, 9, 16, K.SPLIT_F16),
+    (800, 32, 128, 10, K.F16),
+])
+def test_autotuned_shapes_compute_identical_sums(T, A, B, S, kernel):
+    """Every tile shape the autotuner tries computes the same sums in the
+    same order, so the tuned launch is bit-identical to the default one."""
+    x, w = random_case(T, A, B, seed=T + A + B)
+    dims = api.Dimensions(T, F, A, B)
+    api.autotune_clear()
+    default = run_gpu(kernel, x, w, dims, S)
+    dx, dw = api.DeviceArray.from_numpy(x), api.DeviceArray.from_numpy(w)
+    rows = T if S == 0 else api.integrated_time_count(T, S)
+    out = api.DeviceArray.empty((rows, F, B), np.float32)
+    table = api.autotune_packed_beamformer(kernel, None, dx, dw, out, dims, S, repeats=2)
+    n_shapes = {32: 4, 64: 4, 128: 3, 256: 1}[A]
+    assert len(table) == n_shapes * 2 * 3
+    assert all(r["median_ms"] > 0 for r in table)
+    assert np.array_equal(out.to_numpy(), default)  # the winner's output
+    tuned = run_gpu(kernel, x, w, dims, S)             # a later launch takes the winner
+    assert np.array_equal(tuned, default)
+    api.autotune_clear()
+    with pytest.raises(_capi.InvalidArgument, match="autotuner"):
+        api.autotune_packed_beamformer(K.FP32, None, dx, dw, out, dims, S)
