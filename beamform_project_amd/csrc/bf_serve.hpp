@@ -4,9 +4,10 @@
 // The server keeps one voltage ring [R][F][A] (R a power of two, spectrum t
 // at row t mod R), a request table of kServeMaxSlots slots in device memory,
 // and a control block.  A step processes every window of every active slot
-// whose last spectrum lies in (processed, head]: a one-workgroup plan kernel
-// turns the slot table into per-slot item ranges, and one persistent kernel
-// works through the items (slot, channel, run of windows).
+// whose last spectrum lies in (previous head, head]: a one-workgroup plan
+// kernel reads the step's head and turns the slot table into work offsets,
+// and one persistent kernel works through workgroup items (multi-tile
+// windows) and then wave tasks (one-tile windows).
 #pragma once
 
 #include "bf_tracker.hpp"
@@ -35,18 +36,30 @@ struct ServeSlot {
 };
 static_assert(sizeof(ServeSlot) == 64 + 16 * kServeMaxBeams, "ServeSlot layout");
 
+// Plan state: steps planned so far and the head of the previous step.
 struct ServeCtrl {
-    long long head;       // spectra landed and released to this step
-    long long processed;  // head of the previous step
-    long long n_items;    // items of the current step
-    long long reserved;
+    long long step;
+    long long processed;
+    long long reserved[2];
 };
 
-// Per-step work table written by the plan kernel: slot s owns items
-// [off[s], off[s+1]), ordered (run of windows, channel), and its windows
-// [w_lo[s], w_lo[s] + n_win[s]).
+// Heads of the steps, written by the host into pinned host memory before it
+// launches each step: step k releases spectra up to heads[k mod
+// kServeHeadRing].  The host keeps fewer than kServeHeadRing / 2 steps in
+// flight, so an entry is never rewritten before its step has read it.
+inline constexpr int kServeHeadRing = 4096;
+
+// Windows of one wave task of a one-tile-window slot.
+inline constexpr int kServeShortRun = 16;
+
+// Per-step work table written by the plan kernel.  Slots whose windows span
+// several 16-spectrum tiles make workgroup items (run of windows, channel),
+// [off_long[s], off_long[s+1]); slots whose windows fit one tile make wave
+// tasks (run of up to kServeShortRun windows, 16-beam block, channel),
+// [off_short[s], off_short[s+1]).  Channels are fastest in both.
 struct ServeWork {
-    int off[kServeMaxSlots + 1];
+    int off_long[kServeMaxSlots + 1];
+    int off_short[kServeMaxSlots + 1];
     int w_lo[kServeMaxSlots];
     int n_win[kServeMaxSlots];
 };
@@ -59,20 +72,20 @@ struct ServeLaunch {
     const ServeSlot* slots = nullptr;
     ServeCtrl* ctrl = nullptr;
     ServeWork* work = nullptr;
+    const long long* heads = nullptr;  // device view of the pinned head ring
     int grid = 0;  // persistent workgroups of the item kernel
     bool split = true;
 };
 
-// Windows per work item for an S-spectrum window (about 16 tiles per item).
+// Windows per workgroup item for multi-tile windows (about 16 tiles).
 inline int serve_windows_per_item(int S) {
     const int tpw = (S + 15) / 16;
     const int w = 16 / tpw;
     return w < 1 ? 1 : w;
 }
 
-// ctrl->head = head (one tiny kernel; keeps the captured step invariant).
-void launch_serve_set_head(hipStream_t stream, ServeCtrl* ctrl, long long head);
-// The step itself: plan kernel + item kernel (capturable).
+// The step: plan kernel + item kernel (capturable; the head comes from the
+// pinned ring, so the captured launch never changes).
 void launch_serve_step(hipStream_t stream, const ServeLaunch& launch);
 
 }  // namespace bfhip

@@ -164,6 +164,8 @@ struct bf_server {
     ServeSlot* d_slots = nullptr;
     ServeCtrl* d_ctrl = nullptr;
     ServeWork* d_work = nullptr;
+    long long* h_heads = nullptr;  // pinned ring of step heads
+    long long launched = 0;        // steps launched so far
     float* d_pos = nullptr;
     float* d_freqs = nullptr;
     ServeSlot* h_staging = nullptr;  // pinned, one descriptor per slot
@@ -209,6 +211,7 @@ struct bf_server {
                         (void*)d_work, (void*)d_pos, (void*)d_freqs})
             if (p) (void)hipFree(p);
         if (h_staging) (void)hipHostFree(h_staging);
+        if (h_heads) (void)hipHostFree(h_heads);
         if (step_stream) (void)hipStreamDestroy(step_stream);
         if (ingest_stream) (void)hipStreamDestroy(ingest_stream);
     }
@@ -395,8 +398,15 @@ struct bf_server {
         const Commit* c = commit_of(H - 1);
         if (c != nullptr)
             hip_check(hipStreamWaitEvent(step_stream, c->ev, 0), "hipStreamWaitEvent");
+        // Bound the steps in flight so a head-ring entry is never reused
+        // before its step has read it.
+        while (steps.size() >= (size_t)kServeHeadRing / 2) {
+            hip_check(hipEventSynchronize(steps.front().ev_done), "hipEventSynchronize");
+            harvest(false);
+        }
+        h_heads[launched & (kServeHeadRing - 1)] = H;
+        ++launched;
         hip_check(hipEventRecord(st.ev_start, step_stream), "hipEventRecord");
-        launch_serve_set_head(step_stream, d_ctrl, H);
         if (use_graph) {
             if (graph_exec == nullptr) {
                 hip_check(hipStreamBeginCapture(step_stream, hipStreamCaptureModeThreadLocal),
@@ -564,6 +574,14 @@ void create_server(const bf_server_config* c, std::unique_ptr<bf_server>* out) {
                             kServeMaxSlots * sizeof(ServeSlot), hipHostMallocDefault),
               "hipHostMalloc staging");
     std::memset(s->h_staging, 0, kServeMaxSlots * sizeof(ServeSlot));
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&s->h_heads),
+                            kServeHeadRing * sizeof(long long),
+                            hipHostMallocMapped | hipHostMallocCoherent),
+              "hipHostMalloc heads");
+    std::memset(s->h_heads, 0, kServeHeadRing * sizeof(long long));
+    long long* d_heads = nullptr;
+    hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_heads), s->h_heads, 0),
+              "hipHostGetDevicePointer heads");
     // Geometry tables; the defaults are the reference's regular grid of 8
     // columns at 0.6 m and channels 300 MHz + k * 300 kHz
     // (src/geometry.cpp:33-95), evaluated in float as the tracker does.
@@ -602,11 +620,12 @@ void create_server(const bf_server_config* c, std::unique_ptr<bf_server>* out) {
     L.slots = s->d_slots;
     L.ctrl = s->d_ctrl;
     L.work = s->d_work;
+    L.heads = d_heads;
     hipDeviceProp_t prop{};
     int dev = 0;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-    L.grid = std::max(1, prop.multiProcessorCount) * 4;
+    L.grid = std::max(1, prop.multiProcessorCount) * 2;  // resident at 2 workgroups per CU
     L.split = s->split;
     *out = std::move(s);
 }

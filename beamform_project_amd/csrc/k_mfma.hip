@@ -28,7 +28,10 @@ std::map<TuneKey, MfmaConfig>& tune_cache() {
 }
 
 // Beams per wave: as many 8-beam N-tiles as the beam count needs, capped so
-// that the split weights take at most 128 VGPRs per lane.
+// that the split weights take at most 128 VGPRs per lane.  Two LDS stages
+// and chunks sized for 1024 resident workgroups measured fastest on MI355X
+// for the full-shard problem (A=64, B=128, S=10: 1.21 ms against 1.39 ms
+// with one stage); at A=256 the second stage would spill registers.
 MfmaConfig default_config(int A, int B) {
     const int max_nt = A <= 64 ? 4 : (A == 128 ? 2 : 1);
     const int need = (B + 31) / 32;
@@ -36,8 +39,8 @@ MfmaConfig default_config(int A, int B) {
     c.waves = 4;
     c.nt = need <= 1 ? 1 : (need <= 2 ? std::min(2, max_nt) : max_nt);
     if (A == 256) c.nt = 1;
-    c.db = 0;
-    c.resident = 512;
+    c.db = A == 256 ? 0 : 1;
+    c.resident = A == 256 ? 512 : 1024;
     return c;
 }
 
@@ -52,7 +55,7 @@ bool launch_cfg(hipStream_t stream, const MfmaLaunch& L, int A, const MfmaConfig
 }
 
 std::vector<std::pair<int, int>> shapes_for(int A) {
-    if (A <= 64) return {{4, 4}, {2, 8}, {2, 4}, {1, 8}};
+    if (A <= 64) return {{4, 4}, {2, 8}, {2, 4}, {1, 8}, {1, 4}};
     if (A == 128) return {{2, 4}, {1, 8}, {1, 4}};
     return {{1, 4}};
 }

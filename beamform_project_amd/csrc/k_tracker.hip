@@ -8,9 +8,11 @@
 // then contracts like the fixed grid (src/cpu_beamformer.cpp:56-78).
 // Weights are generated inside the kernels, per window, never stored:
 // cos/sin are evaluated in FP64 as the reference does, except that the
-// MFMA kernels reduce the FP64 phase to [-pi, pi] and evaluate it in FP32
-// for windows shorter than 64 spectra (about one float ulp, below the
-// FP16 split's rounding), where weight generation would dominate.
+// MFMA kernels, for windows shorter than 64 spectra (where weight
+// generation would dominate), evaluate every 16th window from the FP64
+// phase reduced to [-pi, pi] in FP32 and rotate it by the small phase
+// change to the windows in between (a few float ulps, below the FP16
+// split's rounding).
 //
 // This translation unit is compiled without FMA contraction so that the
 // direction (float) and phase (double) arithmetic rounds exactly like the
@@ -100,6 +102,19 @@ __device__ __forceinline__ void antenna_position(const Geom& geo, int a, float p
     }
 }
 
+// Positions of all antennas, [A][3], copied into LDS by every workgroup
+// that generates weights (read many times per window).
+template <int A>
+__device__ __forceinline__ void stage_positions(const Geom& geo, float* s_pos) {
+    for (int a = threadIdx.x; a < A; a += blockDim.x) {
+        float p[3];
+        antenna_position(geo, a, p);
+        s_pos[3 * a] = p[0];
+        s_pos[3 * a + 1] = p[1];
+        s_pos[3 * a + 2] = p[2];
+    }
+}
+
 // Channel frequency: a device table, or src/geometry.cpp:79-95.
 __device__ __forceinline__ float channel_frequency(const Geom& geo, int f) {
     if (geo.freqs != nullptr) return geo.freqs[f];
@@ -127,65 +142,162 @@ __device__ __forceinline__ float2 steering_weight_reduced(const float p[3], cons
 }
 
 // Windows of at least this many spectra take the weights from an FP64
-// sincos, as the reference does; shorter windows, where weight generation
-// would dominate, take the reduced FP32 evaluation above.
+// sincos, as the reference does.  Shorter windows, where weight generation
+// would dominate, use anchored weights: every kAnchorWindows-th window (by
+// window index) is an anchor evaluated with steering_weight_reduced, and
+// window w rotates the anchor's phasor by the phase change
+// delta = k * (p . (d_w - d_anchor)), formed in FP32 from the small
+// direction difference.  Each window's weights depend only on w and its
+// anchor, never on how windows are grouped into launches.
 constexpr int kExactWeightMinSpectra = 64;
+constexpr int kAnchorWindows = 16;
 
-// B operand of one wave for one window.  Lanes c and c^1 hold the Re and
-// Im columns of the same beam over the same antennas, so each computes
-// half of the weights and takes the other half from its partner by DPP.
-template <int A, int NT, bool SPLIT, int DIRSRC>
-__device__ __forceinline__ void window_weights(const DirArgs& da, const Geom& geo,
-                                               double k, int w, int beam0, int lane,
-                                               bool exact, bf_half8 (&bhi)[NT][A / 16],
-                                               bf_half8 (&blo)[NT][A / 16]) {
+// Anchor phasors of this lane's own weights and the anchor's directions.
+template <int A, int NT>
+struct WeightAnchor {
+    float2 own[NT][A / 16][2];
+    float dir[NT][3];
+    int window;  // -1: none yet
+};
+
+// (cos, sin)(phi + delta) from (cos, sin)(phi) = z; |delta| < 2^-10 uses
+// cos delta = 1 - delta^2/2, sin delta = delta - delta^3/6 (the next terms
+// are below 2^-40 relative).
+__device__ __forceinline__ float2 rotate_phasor(float2 z, float delta) {
+    float sd, cd;
+    if (fabsf(delta) < 0x1p-10f) {
+        const float d2 = delta * delta;
+        cd = fmaf(d2, -0.5f, 1.0f);
+        sd = fmaf(delta * d2, -1.0f / 6.0f, delta);
+    } else {
+        sincosf(delta, &sd, &cd);
+    }
+    return make_float2(fmaf(z.x, cd, -(z.y * sd)), fmaf(z.y, cd, z.x * sd));
+}
+
+// hi/lo FP16 pairs of one complex weight, packed (re in the low half):
+// ph = (f16(wr), f16(wi)), pl = (f16(wr - hi), f16(wi - hi)).
+__device__ __forceinline__ void split_pack(float2 v, bool split, unsigned& ph, unsigned& pl) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const _Float16 hr = (_Float16)v.x;
+    const _Float16 hi = (_Float16)v.y;
+    ph = __builtin_bit_cast(unsigned, h2{hr, hi});
+    pl = 0u;
+    if (split) {
+        const _Float16 lr = (_Float16)(v.x - (float)hr);
+        const _Float16 li = (_Float16)(v.y - (float)hi);
+        pl = __builtin_bit_cast(unsigned, h2{lr, li});
+    }
+}
+
+__device__ __forceinline__ unsigned dpp_swap_pairs_u(unsigned v) {
+    return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);
+}
+
+// B operand of one wave for window w.  Lane c holds column c of each
+// 8-beam N-tile (Re of beam c/2 for even c, Im for odd c) over antennas
+// g*(A/4) + 4s + j.  Lanes c and c^1 need the same complex weights, so each
+// evaluates two of every four (j = 2*comp_im + jj), splits them into hi/lo
+// FP16 pairs and takes the partner's two pairs by DPP; a Re column then
+// holds (w_re, -w_im) (sign flip of the high half) and an Im column
+// (w_im, w_re) (halves swapped).  Without ANCHORED (or with `exact`) every
+// weight comes from an FP64 sincos; with ANCHORED the anchor window's
+// phasors are kept in `anc` and rotated to window w.  Positions come from
+// the LDS copy s_pos.
+template <int A, int NT, bool SPLIT, int DIRSRC, bool ANCHORED>
+__device__ __forceinline__ void window_operands(const DirArgs& da, const float* s_pos,
+                                                double k, int w, int beam0, int lane,
+                                                bool exact, WeightAnchor<A, NT>& anc,
+                                                bf_half8 (&bhi)[NT][A / 16],
+                                                bf_half8 (&blo)[NT][A / 16]) {
     constexpr int KS = A / 16;
     const int c = lane & 15;
     const int g = lane >> 4;
     const int comp_im = c & 1;
+    const bool use_anchor = ANCHORED && !exact;
+    const int wa = w - w % kAnchorWindows;
+    if (use_anchor && anc.window != wa) {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int b = beam0 + nt * 8 + (c >> 1);
+            const bool live = b < da.B;
+            float d[3] = {0.0f, 0.0f, 1.0f};
+            if (live) beam_direction<DIRSRC>(da, wa, b, d);
+            anc.dir[nt][0] = d[0];
+            anc.dir[nt][1] = d[1];
+            anc.dir[nt][2] = d[2];
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj) {
+                    anc.own[nt][s][jj] = make_float2(0.0f, 0.0f);
+                    if (live) {
+                        const float* p = s_pos + 3 * (g * (A / 4) + 4 * s + 2 * comp_im + jj);
+                        anc.own[nt][s][jj] = steering_weight_reduced(p, d, k);
+                    }
+                }
+            }
+        }
+        anc.window = wa;
+    }
+    const float kf = (float)k;
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
         const int b = beam0 + nt * 8 + (c >> 1);
         const bool live = b < da.B;
         float d[3] = {0.0f, 0.0f, 1.0f};
         if (live) beam_direction<DIRSRC>(da, w, b, d);
+        const bool rotate = use_anchor && live && w != wa;
+        float dd[3] = {0.0f, 0.0f, 0.0f};
+        if (rotate) {
+            dd[0] = d[0] - anc.dir[nt][0];
+            dd[1] = d[1] - anc.dir[nt][1];
+            dd[2] = d[2] - anc.dir[nt][2];
+        }
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-            float2 own[2];
-            float2 other[2];
+            unsigned ph[2], pl[2];
 #pragma unroll
             for (int jj = 0; jj < 2; ++jj) {
-                const int a = g * (A / 4) + 4 * s + 2 * comp_im + jj;
-                own[jj] = make_float2(0.0f, 0.0f);
-                if (live) {
-                    float p[3];
-                    antenna_position(geo, a, p);
-                    own[jj] = exact ? steering_weight(p, d, k)
-                                    : steering_weight_reduced(p, d, k);
+                const float* p = s_pos + 3 * (g * (A / 4) + 4 * s + 2 * comp_im + jj);
+                float2 v;
+                if (use_anchor) {
+                    v = anc.own[nt][s][jj];
+                    if (rotate) {
+                        float t = p[0] * dd[0];
+                        t = fmaf(p[1], dd[1], t);
+                        t = fmaf(p[2], dd[2], t);
+                        v = rotate_phasor(v, kf * t);
+                    }
+                } else {
+                    v = make_float2(0.0f, 0.0f);
+                    if (live) v = steering_weight(p, d, k);
                 }
+                split_pack(v, SPLIT, ph[jj], pl[jj]);
             }
+            unsigned qh[2], ql[2];  // partner's pairs
 #pragma unroll
             for (int jj = 0; jj < 2; ++jj) {
-                other[jj].x = dpp_swap_pairs(own[jj].x);
-                other[jj].y = dpp_swap_pairs(own[jj].y);
+                qh[jj] = dpp_swap_pairs_u(ph[jj]);
+                if (SPLIT) ql[jj] = dpp_swap_pairs_u(pl[jj]);
             }
-            bf_half8 h, l;
+            // Re column: (re, -im) = sign flip of the high half; Im column:
+            // (im, re) = halves rotated.
+            const unsigned rot = comp_im ? 16u : 0u;
+            const unsigned flip = comp_im ? 0u : 0x80000000u;
+            bf_uint4v uh, ul;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float2 v = (j >> 1) == comp_im ? own[j & 1] : other[j & 1];
-                const float e0 = comp_im ? v.y : v.x;
-                const float e1 = comp_im ? v.x : -v.y;
-                const _Float16 h0 = (_Float16)e0;
-                const _Float16 h1 = (_Float16)e1;
-                h[2 * j] = h0;
-                h[2 * j + 1] = h1;
+                const bool mine = (j >> 1) == comp_im;
+                const unsigned xh = mine ? ph[j & 1] : qh[j & 1];
+                uh[j] = __builtin_amdgcn_alignbit(xh, xh, rot) ^ flip;
                 if (SPLIT) {
-                    l[2 * j] = (_Float16)(e0 - (float)h0);
-                    l[2 * j + 1] = (_Float16)(e1 - (float)h1);
+                    const unsigned xl = mine ? pl[j & 1] : ql[j & 1];
+                    ul[j] = __builtin_amdgcn_alignbit(xl, xl, rot) ^ flip;
                 }
             }
-            bhi[nt][s] = h;
-            if (SPLIT) blo[nt][s] = l;
+            bhi[nt][s] = __builtin_bit_cast(bf_half8, uh);
+            if (SPLIT) blo[nt][s] = __builtin_bit_cast(bf_half8, ul);
         }
     }
 }
@@ -211,16 +323,19 @@ struct OutTarget {
     int f;
 };
 
-// Windows [w_begin, w_end) of one channel, for the NT*8 beams of this wave
-// starting at beam0.  Entered by the whole workgroup (kThreads threads):
-// stages of ST window-aligned tiles of 16 spectra go through LDS in the
-// A-operand layout, the next stage is held in registers meanwhile, and the
-// B operand (this window's weights) is regenerated at each window's first
-// tile.
-template <int A, int NT, bool SPLIT, bool INTEGRATE, int DIRSRC, int ST>
-__device__ __forceinline__ void tracker_windows(const RowSource& rs, const Geom& geo,
+// Windows [w_begin, w_end) of one channel for the beams wg_beam0 ..
+// wg_beam0 + kWaves*NT*8 - 1.  Entered by the whole workgroup (kThreads
+// threads): stages of ST window-aligned tiles of 16 spectra go through LDS
+// in the A-operand layout, the next stage is held in registers meanwhile,
+// and the B operand (a window's weights) is regenerated at each window's
+// first tile.  Each wave owns one block of NT*8 beams; when a window fits
+// one tile and fewer than kWaves blocks hold beams, the idle waves share
+// the blocks and the tiles of each stage are dealt out among a block's
+// waves, so every wave generates weights.
+template <int A, int NT, bool SPLIT, bool INTEGRATE, int DIRSRC, int ST, bool ANCHORED>
+__device__ __forceinline__ void tracker_windows(const RowSource& rs, const float* s_pos,
                                                 const DirArgs& da, const OutTarget& ot, double k,
-                                                int w_begin, int w_end, int tpw, int beam0) {
+                                                int w_begin, int w_end, int tpw, int wg_beam0) {
     constexpr int KS = A / 16;
     constexpr int TILE_BYTES = KS * kWaveSize * 16;
     constexpr int DW_ROW = A / 4;
@@ -235,6 +350,12 @@ __device__ __forceinline__ void tracker_windows(const RowSource& rs, const Geom&
     const int f = ot.f;
     const int tid = threadIdx.x;
     const int lane = tid & (kWaveSize - 1);
+    const int wave = tid / kWaveSize;
+    int blocks = (B - wg_beam0 + NT * 8 - 1) / (NT * 8);  // blocks holding beams
+    blocks = blocks < 1 ? 1 : (blocks > kWaves ? kWaves : blocks);
+    const int per_block = tpw == 1 ? kWaves / blocks : 1;  // waves sharing a block
+    const int beam0 = wg_beam0 + (wave / per_block) * NT * 8;
+    const int tile_first = wave % per_block;
     const bool has_beams = beam0 < B;
     const int q_end = (w_end - w_begin) * tpw;  // tiles of this call
 
@@ -284,6 +405,8 @@ __device__ __forceinline__ void tracker_windows(const RowSource& rs, const Geom&
 
     bf_half8 bhi[NT][KS];
     bf_half8 blo[NT][KS];
+    WeightAnchor<A, NT> anc;
+    anc.window = -1;
     float win[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) win[nt] = 0.0f;
@@ -298,14 +421,14 @@ __device__ __forceinline__ void tracker_windows(const RowSource& rs, const Geom&
         if (q0 + ST < q_end) load_stage(q0 + ST);
         if (!has_beams) continue;
 #pragma unroll 1
-        for (int tl = 0; tl < ST && q0 + tl < q_end; ++tl) {
+        for (int tl = tile_first; tl < ST && q0 + tl < q_end; tl += per_block) {
             const int qt = q0 + tl;
             const int wl = qt / tpw;        // window within this call
             const int jt = qt - wl * tpw;   // tile within the window
             const int w = w_begin + wl;
             if (jt == 0)
-                window_weights<A, NT, SPLIT, DIRSRC>(da, geo, k, w, beam0, lane,
-                                                     S >= kExactWeightMinSpectra, bhi, blo);
+                window_operands<A, NT, SPLIT, DIRSRC, ANCHORED>(
+                    da, s_pos, k, w, beam0, lane, S >= kExactWeightMinSpectra, anc, bhi, blo);
             bf_half8 av[KS];
 #pragma unroll
             for (int s = 0; s < KS; ++s)
@@ -368,25 +491,26 @@ __device__ __forceinline__ void tracker_windows(const RowSource& rs, const Geom&
     }
 }
 
-template <int A, int NT, bool SPLIT, bool INTEGRATE, int DIRSRC, int ST>
+template <int A, int NT, bool SPLIT, bool INTEGRATE, int DIRSRC, int ST, bool ANCHORED>
 __global__ __launch_bounds__(kThreads) void tracker_mfma_kernel(
     const uint8_t* __restrict__ x, Geom geo, DirArgs da, float* __restrict__ out, int T, int F,
     int tpw, int n_windows, int windows_per_chunk, int n_items, int per_xcd) {
-    const int wave = threadIdx.x / kWaveSize;
+    __shared__ float s_pos[3 * A];
     const int item = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
     if (item >= n_items) return;
+    stage_positions<A>(geo, s_pos);  // visible after tracker_windows' first barrier
     const int n_chunks = n_items / F;
     const int f = item / n_chunks;
     const int chunk = item - f * n_chunks;
-    const int beam0 = (blockIdx.y * kWaves + wave) * NT * 8;
+    const int wg_beam0 = blockIdx.y * kWaves * NT * 8;
     const double k = wave_number(channel_frequency(geo, f));
     const int w_begin = chunk * windows_per_chunk;
     int w_end = w_begin + windows_per_chunk;
     w_end = w_end < n_windows ? w_end : n_windows;
     const RowSource rs{x + (size_t)f * A, (size_t)F * A, -1L, 0L, (long)T};
     const OutTarget ot{out, 0L, F, f};
-    tracker_windows<A, NT, SPLIT, INTEGRATE, DIRSRC, ST>(rs, geo, da, ot, k, w_begin, w_end, tpw,
-                                                         beam0);
+    tracker_windows<A, NT, SPLIT, INTEGRATE, DIRSRC, ST, ANCHORED>(rs, s_pos, da, ot, k, w_begin,
+                                                                   w_end, tpw, wg_beam0);
 }
 
 // ------------------------------------------------------------ FP32 tracker
@@ -477,21 +601,26 @@ __global__ __launch_bounds__(256) void tracker_fp32_kernel(
 }
 
 // ------------------------------------------------------------ serving loop
-// Plan: one thread per request slot finds the windows whose last spectrum
-// lies in (processed, head] and the items they make; a block-wide scan
-// turns the counts into item offsets.  The previous head is read before
-// the barrier and replaced after it, so each step sees its own range.
+// Plan: one thread per request slot reads the step's head, finds the
+// windows whose last spectrum lies in (processed, head] and the work they
+// make; block scans turn the counts into offsets.  The previous head is
+// read before the first barrier and replaced after the last one.
 __global__ __launch_bounds__(kServeMaxSlots) void serve_plan_kernel(
     const ServeSlot* __restrict__ slots, ServeCtrl* __restrict__ ctrl,
-    ServeWork* __restrict__ work, int F) {
-    __shared__ int scan[kServeMaxSlots];
+    ServeWork* __restrict__ work, const long long* heads, int F) {
+    __shared__ int scan_l[kServeMaxSlots];
+    __shared__ int scan_s[kServeMaxSlots];
     const int s = threadIdx.x;
-    const long long H = ctrl->head;
+    const long long step = ctrl->step;
+    // system-scope load: the host wrote this entry just before the launch
+    const long long H = __hip_atomic_load(heads + (step & (kServeHeadRing - 1)),
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     const long long P = ctrl->processed;
     const ServeSlot& sl = slots[s];
     int n_win = 0;
     int w_lo = 0;
-    int items = 0;
+    int n_long = 0;
+    int n_short = 0;
     if (sl.active) {
         const long long S = sl.S;
         const long long t0 = sl.t0;
@@ -503,57 +632,173 @@ __global__ __launch_bounds__(kServeMaxSlots) void serve_plan_kernel(
         if (hi > lo) {
             n_win = (int)(hi - lo);
             w_lo = (int)lo;
-            items = F * ((n_win + sl.wpi - 1) / sl.wpi);
+            if (S <= 16)
+                n_short = F * ((sl.B + 15) / 16) * ((n_win + kServeShortRun - 1) / kServeShortRun);
+            else
+                n_long = F * ((n_win + sl.wpi - 1) / sl.wpi);
         }
     }
-    scan[s] = items;
+    scan_l[s] = n_long;
+    scan_s[s] = n_short;
     __syncthreads();
     for (int d = 1; d < kServeMaxSlots; d <<= 1) {
-        const int v = s >= d ? scan[s - d] : 0;
+        const int vl = s >= d ? scan_l[s - d] : 0;
+        const int vs = s >= d ? scan_s[s - d] : 0;
         __syncthreads();
-        scan[s] += v;
+        scan_l[s] += vl;
+        scan_s[s] += vs;
         __syncthreads();
     }
-    work->off[s] = scan[s] - items;
+    work->off_long[s] = scan_l[s] - n_long;
+    work->off_short[s] = scan_s[s] - n_short;
     work->w_lo[s] = w_lo;
     work->n_win[s] = n_win;
     if (s == kServeMaxSlots - 1) {
-        work->off[kServeMaxSlots] = scan[s];
-        ctrl->n_items = scan[s];
+        work->off_long[kServeMaxSlots] = scan_l[s];
+        work->off_short[kServeMaxSlots] = scan_s[s];
         ctrl->processed = H;
+        ctrl->step = step + 1;
     }
 }
 
-// Items: persistent workgroups stride over the step's items.  Item i
-// belongs to the last slot s with off[s] <= i; within the slot it is
-// (run of windows, channel) with channels fastest.  Four waves cover up to
-// 64 beams (16 per wave).
+// Last slot s with off[s] <= i (off non-decreasing, off[0] = 0 <= i < off[n]).
+__device__ __forceinline__ int owner_slot(const int* off, int i) {
+    int lo = 0;
+    int hi = kServeMaxSlots;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// One wave task of a one-tile-window slot: windows [w_begin, w_end) of
+// channel f for the 16 beams from beam0.  The wave works alone: each lane
+// loads its A-operand dwords (row lane % 16, antenna group lane / 16)
+// straight from the ring, one window ahead, and the arithmetic -- anchored
+// weights, MFMA order, window sum and cross-lane fold -- is that of
+// tracker_windows for one-tile windows, so results are bit-identical to the
+// multi-beam tracker.
+template <int A, bool SPLIT>
+__device__ __forceinline__ void serve_short_task(const uint8_t* __restrict__ ring,
+                                                 long long ring_mask, int F,
+                                                 const float* s_pos, const ServeSlot* sl,
+                                                 int f, int beam0, int w_begin, int w_end,
+                                                 double k) {
+    constexpr int KS = A / 16;
+    constexpr int NT = 2;
+    const int lane = threadIdx.x & (kWaveSize - 1);
+    const int col = lane & 15;
+    const int G = lane >> 4;
+    const int S = sl->S;
+    const int B = sl->B;
+    const long t0 = (long)sl->t0;
+    const long te = (long)sl->t_end;
+    const long out_windows = (long)sl->out_windows;
+    float* out = sl->out;
+    const DirArgs da{nullptr, &sl->traj[0][0], 0L, S, B};
+    const size_t row_stride = (size_t)F * A;
+    const uint8_t* base = ring + (size_t)f * A + (size_t)G * (A / 4);
+    const int row = col;  // A operand: lane holds row lane % 16, group lane / 16
+
+    WeightAnchor<A, NT> anc;
+    anc.window = -1;
+    // A-operand dwords of window w for this lane (zero past the window)
+    unsigned cur[KS], nxt[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) cur[s] = nxt[s] = 0u;
+    {
+        const long t = t0 + (long)w_begin * S + row;
+        if (row < S && t < te) {
+            const uint8_t* p = base + (size_t)(t & ring_mask) * row_stride;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) cur[s] = *reinterpret_cast<const unsigned*>(p + 4 * s);
+        }
+    }
+#pragma unroll 1
+    for (int w = w_begin; w < w_end; ++w) {
+        {
+            const long t = t0 + (long)(w + 1) * S + row;
+            const bool ok = w + 1 < w_end && row < S && t < te;
+            const uint8_t* p = base + (size_t)(t & ring_mask) * row_stride;
+#pragma unroll
+            for (int s = 0; s < KS; ++s)
+                nxt[s] = ok ? *reinterpret_cast<const unsigned*>(p + 4 * s) : 0u;
+        }
+        bf_half8 bhi[NT][KS];
+        bf_half8 blo[NT][KS];
+        window_operands<A, NT, SPLIT, kDirTrajectory, true>(da, s_pos, k, w, beam0, lane,
+                                                            false, anc, bhi, blo);
+        bf_half8 av[KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            unsigned h2[4];
+            bf_int4x4_to_half2x4(cur[s], h2);
+            av[s] = __builtin_bit_cast(bf_half8, bf_uint4v{h2[0], h2[1], h2[2], h2[3]});
+        }
+        bf_float4 acc[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[nt] = bf_float4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[s], bhi[nt][s], acc[nt], 0, 0,
+                                                                 0);
+            if (SPLIT) {
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[s], blo[nt][s], acc[nt],
+                                                                     0, 0, 0);
+            }
+        }
+        const long slot = out_windows > 0 ? (long)w % out_windows : (long)w;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            float v = 0.0f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v = fmaf(acc[nt][r], acc[nt][r], v);
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            v += dpp_swap_pairs(v);
+            const int b = beam0 + nt * 8 + (col >> 1);
+            if (G == 0 && (col & 1) == 0 && b < B) out[((size_t)slot * F + f) * B + b] = v;
+        }
+#pragma unroll
+        for (int s = 0; s < KS; ++s) cur[s] = nxt[s];
+    }
+}
+
+// Items: persistent workgroups first share the workgroup items of
+// multi-tile-window slots (four waves cover up to 64 beams), then each wave
+// takes wave tasks of one-tile-window slots on its own.
 template <int A, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void serve_items_kernel(
     const uint8_t* __restrict__ ring, long long ring_mask, int F, Geom geo,
     const ServeSlot* __restrict__ slots, const ServeWork* __restrict__ work) {
-    __shared__ int off[kServeMaxSlots + 1];
-    for (int i = threadIdx.x; i <= kServeMaxSlots; i += kThreads) off[i] = work->off[i];
+    __shared__ int off_l[kServeMaxSlots + 1];
+    __shared__ int off_s[kServeMaxSlots + 1];
+    __shared__ float s_pos[3 * A];
+    for (int i = threadIdx.x; i <= kServeMaxSlots; i += kThreads) {
+        off_l[i] = work->off_long[i];
+        off_s[i] = work->off_short[i];
+    }
+    stage_positions<A>(geo, s_pos);
     __syncthreads();
-    const int n_items = off[kServeMaxSlots];
-    const int wave = threadIdx.x / kWaveSize;
-    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
-        int lo = 0;
-        int hi = kServeMaxSlots;  // off[lo] <= item < off[hi]
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (off[mid] <= item)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        const ServeSlot* sl = slots + lo;
-        const int local = item - off[lo];
+    const int n_long = off_l[kServeMaxSlots];
+    const int n_short = off_s[kServeMaxSlots];
+    for (int item = blockIdx.x; item < n_long; item += gridDim.x) {
+        const int sidx = owner_slot(off_l, item);
+        const ServeSlot* sl = slots + sidx;
+        const int local = item - off_l[sidx];
         const int run = local / F;
         const int f = local - run * F;
         const int S = sl->S;
-        const int w_first = work->w_lo[lo];
-        const int w_last = w_first + work->n_win[lo];
+        const int w_first = work->w_lo[sidx];
+        const int w_last = w_first + work->n_win[sidx];
         const int w_begin = w_first + run * sl->wpi;
         const int w_end = w_begin + sl->wpi < w_last ? w_begin + sl->wpi : w_last;
         const DirArgs da{nullptr, &sl->traj[0][0], 0L, S, sl->B};
@@ -561,16 +806,31 @@ __global__ __launch_bounds__(kThreads) void serve_items_kernel(
                            (long)sl->t_end};
         const OutTarget ot{sl->out, (long)sl->out_windows, F, f};
         const double k = wave_number(channel_frequency(geo, f));
-        tracker_windows<A, 2, SPLIT, true, kDirTrajectory, 4>(rs, geo, da, ot, k, w_begin, w_end,
-                                                              (S + 15) / 16, wave * 16);
+        tracker_windows<A, 2, SPLIT, true, kDirTrajectory, 4, true>(
+            rs, s_pos, da, ot, k, w_begin, w_end, (S + 15) / 16, 0);
+    }
+    const int wave = threadIdx.x / kWaveSize;
+    for (int task = blockIdx.x * kWaves + wave; task < n_short; task += gridDim.x * kWaves) {
+        const int sidx = owner_slot(off_s, task);
+        const ServeSlot* sl = slots + sidx;
+        const int local = task - off_s[sidx];
+        const int blocks = (sl->B + 15) / 16;
+        const int f = local % F;
+        const int rest = local / F;
+        const int block = rest % blocks;
+        const int run = rest / blocks;
+        const int w_first = work->w_lo[sidx];
+        const int w_last = w_first + work->n_win[sidx];
+        const int w_begin = w_first + run * kServeShortRun;
+        const int w_end =
+            w_begin + kServeShortRun < w_last ? w_begin + kServeShortRun : w_last;
+        const double k = wave_number(channel_frequency(geo, f));
+        serve_short_task<A, SPLIT>(ring, ring_mask, F, s_pos, sl, f, block * 16, w_begin, w_end,
+                                   k);
     }
 }
 
-__global__ void serve_set_head_kernel(ServeCtrl* ctrl, long long head) {
-    if (threadIdx.x == 0) ctrl->head = head;
-}
-
-template <int A, int NT, bool SPLIT, bool INTEGRATE, int DIRSRC>
+template <int A, int NT, bool SPLIT, bool INTEGRATE, int DIRSRC, bool ANCHORED>
 void launch_mfma_nt(hipStream_t stream, const TrackerLaunch& L, const DirArgs& da) {
     const int tpw = (int)ceil_div((size_t)L.S, 16);
     constexpr int ST = 4;
@@ -582,22 +842,33 @@ void launch_mfma_nt(hipStream_t stream, const TrackerLaunch& L, const DirArgs& d
     const int n_items = n_chunks * (int)L.F;
     const int per_xcd = (int)ceil_div((size_t)n_items, kXcds);
     const int beam_blocks = (int)ceil_div(L.B, (size_t)(kWaves * NT * 8));
-    hipLaunchKernelGGL((tracker_mfma_kernel<A, NT, SPLIT, INTEGRATE, DIRSRC, ST>),
+    hipLaunchKernelGGL((tracker_mfma_kernel<A, NT, SPLIT, INTEGRATE, DIRSRC, ST, ANCHORED>),
                        dim3((unsigned)(per_xcd * kXcds), (unsigned)beam_blocks), dim3(kThreads),
                        0, stream, L.x, L.geo, da, L.out, (int)L.T, (int)L.F, tpw,
                        n_windows, wpc, n_items, per_xcd);
 }
 
+// Beams per wave: as many 8-beam N-tiles as the beam count needs, up to 4
+// (2 at A=128, 1 at A=256) with FP64-sincos weights; short windows keep
+// anchor phasors in registers and take at most 2.
 template <int A, bool SPLIT, bool INTEGRATE, int DIRSRC>
 void launch_mfma_a(hipStream_t stream, const TrackerLaunch& L, const DirArgs& da) {
     constexpr int kMaxNT = A <= 64 ? 4 : (A == 128 ? 2 : 1);
+    constexpr int kMaxNT2 = kMaxNT >= 2 ? 2 : 1;
     const size_t need = ceil_div(L.B, (size_t)(kWaves * 8));
+    if (L.S < (size_t)kExactWeightMinSpectra) {
+        if (need <= 1 || kMaxNT2 == 1)
+            launch_mfma_nt<A, 1, SPLIT, INTEGRATE, DIRSRC, true>(stream, L, da);
+        else
+            launch_mfma_nt<A, kMaxNT2, SPLIT, INTEGRATE, DIRSRC, true>(stream, L, da);
+        return;
+    }
     if (need <= 1 || kMaxNT == 1)
-        launch_mfma_nt<A, 1, SPLIT, INTEGRATE, DIRSRC>(stream, L, da);
+        launch_mfma_nt<A, 1, SPLIT, INTEGRATE, DIRSRC, false>(stream, L, da);
     else if (need <= 2 || kMaxNT == 2)
-        launch_mfma_nt<A, (kMaxNT >= 2 ? 2 : 1), SPLIT, INTEGRATE, DIRSRC>(stream, L, da);
+        launch_mfma_nt<A, kMaxNT2, SPLIT, INTEGRATE, DIRSRC, false>(stream, L, da);
     else
-        launch_mfma_nt<A, kMaxNT, SPLIT, INTEGRATE, DIRSRC>(stream, L, da);
+        launch_mfma_nt<A, kMaxNT, SPLIT, INTEGRATE, DIRSRC, false>(stream, L, da);
 }
 
 template <bool SPLIT, bool INTEGRATE, int DIRSRC>
@@ -661,14 +932,9 @@ void launch_tracker(hipStream_t stream, const TrackerLaunch& L) {
     hip_check(hipGetLastError(), "tracker kernel launch");
 }
 
-void launch_serve_set_head(hipStream_t stream, ServeCtrl* ctrl, long long head) {
-    hipLaunchKernelGGL(serve_set_head_kernel, dim3(1), dim3(64), 0, stream, ctrl, head);
-    hip_check(hipGetLastError(), "serve head launch");
-}
-
 void launch_serve_step(hipStream_t stream, const ServeLaunch& L) {
     hipLaunchKernelGGL(serve_plan_kernel, dim3(1), dim3(kServeMaxSlots), 0, stream, L.slots,
-                       L.ctrl, L.work, (int)L.F);
+                       L.ctrl, L.work, L.heads, (int)L.F);
 #define BF_SERVE(A_, SPLIT_)                                                                 \
     hipLaunchKernelGGL((serve_items_kernel<A_, SPLIT_>), dim3((unsigned)L.grid), dim3(kThreads), \
                        0, stream, L.ring, L.ring_mask, (int)L.F, L.geo, L.slots, L.work)

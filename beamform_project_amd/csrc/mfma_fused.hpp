@@ -409,8 +409,9 @@ void launch_split_db(hipStream_t stream, const MfmaLaunch& L, const MfmaConfig& 
 // Instantiated shapes per antenna count: (NT, waves) pairs whose split
 // weights fit in at most 128 VGPRs per lane and that compile without
 // register spills for the common modes.
-#define BF_MFMA_SHAPES_32_64(A_) \
-    BF_SHAPE(A_, 4, 4) BF_SHAPE(A_, 2, 8) BF_SHAPE(A_, 2, 4) BF_SHAPE(A_, 1, 8)
+#define BF_MFMA_SHAPES_32_64(A_)                                            \
+    BF_SHAPE(A_, 4, 4) BF_SHAPE(A_, 2, 8) BF_SHAPE(A_, 2, 4) BF_SHAPE(A_, 1, 8) \
+    BF_SHAPE(A_, 1, 4)
 #define BF_MFMA_SHAPES_128 BF_SHAPE(128, 2, 4) BF_SHAPE(128, 1, 8) BF_SHAPE(128, 1, 4)
 #define BF_MFMA_SHAPES_256 BF_SHAPE(256, 1, 4)
 

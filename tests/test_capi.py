@@ -101,3 +101,23 @@ def test_server_config_is_validated_before_any_device_call():
         cfg = serve.ServerConfig(**{**base, **bad})
         with pytest.raises(_capi.InvalidArgument, match=msg):
             serve.TrackerServer(cfg)
+
+
+def test_default_mfma_shapes_are_instantiated():
+    # Every (beams per wave, waves) the dispatcher may pick by default must
+    # exist for its antenna count: checked by the tuner's candidate lists in
+    # k_mfma.hip against the instantiation macros in mfma_fused.hpp.
+    src = open(os.path.join(ROOT, "beamform_project_amd", "csrc", "mfma_fused.hpp")).read()
+    tuner = open(os.path.join(ROOT, "beamform_project_amd", "csrc", "k_mfma.hip")).read()
+    inst = {}
+    for name, body in re.findall(r"#define (BF_MFMA_SHAPES_\w+)(?:\(A_\))?\s*((?:.*\\\n)*.*)",
+                                 src):
+        inst[name] = set(re.findall(r"BF_SHAPE\(\w+, (\d), (\d)\)", body))
+    small = inst["BF_MFMA_SHAPES_32_64"]
+    assert ("1", "4") in small and ("4", "4") in small and ("2", "4") in small
+    assert ("2", "4") in inst["BF_MFMA_SHAPES_128"] and ("1", "4") in inst["BF_MFMA_SHAPES_128"]
+    assert ("1", "4") in inst["BF_MFMA_SHAPES_256"]
+    lists = re.findall(r"return \{(\{\d, \d\}(?:, \{\d, \d\})*)\};", tuner)
+    tuned = [set(re.findall(r"\{(\d), (\d)\}", l)) for l in lists]
+    assert tuned[0] == small and tuned[1] == inst["BF_MFMA_SHAPES_128"]
+    assert tuned[2] == inst["BF_MFMA_SHAPES_256"]

@@ -353,7 +353,7 @@ def test_autotuned_shapes_compute_identical_sums(T, A, B, S, kernel):
     rows = T if S == 0 else api.integrated_time_count(T, S)
     out = api.DeviceArray.empty((rows, F, B), np.float32)
     table = api.autotune_packed_beamformer(kernel, None, dx, dw, out, dims, S, repeats=2)
-    n_shapes = {32: 4, 64: 4, 128: 3, 256: 1}[A]
+    n_shapes = {32: 5, 64: 5, 128: 3, 256: 1}[A]
     assert len(table) == n_shapes * 2 * 3
     assert all(r["median_ms"] > 0 for r in table)
     assert np.array_equal(out.to_numpy(), default)  # the winner's output
