@@ -14,8 +14,8 @@
 //      linear array, steering a_m = exp(-j k x_m sin(theta)) (:96-106).
 //      For a ULA a^H Q a = sum_d q_d exp(j d k dx sin(theta)) with q_d the
 //      sum of Q's d-th diagonal, so each angle costs O(M), not O(M^2).
-//   3. estimate_kernel: broadband spectra summed over channels, local
-//      maxima, 3-point parabolic refinement.
+//   3. broadband_kernel + peaks_kernel: broadband spectra summed over
+//      channels, local maxima, 3-point parabolic refinement.
 //
 // MFMA mapping: with y = [16 re, 16 im] int8 components per antenna (the
 // nibble placed in the top half of a byte is 16x the signed value), the
@@ -54,7 +54,12 @@ __global__ __launch_bounds__(kThreads) void covariance_kernel(
     constexpr int PER_WAVE = (TILES + 3) / 4;
     __shared__ __attribute__((aligned(16))) unsigned char tr[C * kTStride];
     __shared__ int g[C * (C + 1)];  // G, row stride C+1
-    const int f = blockIdx.x;
+    // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
+    // contiguous range of channels so that channels f and f+1, which share
+    // the 128-byte lines of every packed row, meet in one L2.
+    const int per_xcd = (F + 7) / 8;
+    const int f = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (f >= F) return;  // whole workgroup
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -298,64 +303,114 @@ __global__ __launch_bounds__(kThreads) void fold_scan_kernel(
     }
 }
 
-// Broadband spectra (sum over channels) and peak picking, one workgroup.
-__global__ __launch_bounds__(kThreads) void estimate_kernel(
+// Broadband spectra: sum over channels.  A workgroup covers 64 angles; its
+// four waves each sum a quarter of the channels (f = part, part + 4, ...)
+// and the four partial sums are added in a fixed order.
+__global__ __launch_bounds__(kThreads) void broadband_kernel(
     const double* __restrict__ capon, const double* __restrict__ bartlett, int F, int n_theta,
-    const double* __restrict__ theta, int max_peaks, double* __restrict__ broadband /* [2][n] */,
-    double* __restrict__ peaks /* [2][max_peaks][2] theta, power */,
-    int* __restrict__ n_peaks /* [2] */) {
-    const int tid = threadIdx.x;
-    for (int q = tid; q < n_theta; q += kThreads) {
-        double sc = 0.0, sb = 0.0;
-        for (int f = 0; f < F; ++f) {
+    double* __restrict__ broadband /* [2][n] */) {
+    __shared__ double part_c[4][64];
+    __shared__ double part_b[4][64];
+    const int ql = threadIdx.x & 63;
+    const int part = threadIdx.x >> 6;
+    const int q = blockIdx.x * 64 + ql;
+    double sc = 0.0, sb = 0.0;
+    if (q < n_theta) {
+        for (int f = part; f < F; f += 4) {
             sc += capon[(size_t)f * n_theta + q];
             sb += bartlett[(size_t)f * n_theta + q];
         }
-        broadband[q] = sc;
-        broadband[n_theta + q] = sb;
     }
+    part_c[part][ql] = sc;
+    part_b[part][ql] = sb;
     __syncthreads();
-    if (tid < 2) {
-        const double* P = broadband + tid * n_theta;
-        double* out = peaks + (size_t)tid * max_peaks * 2;
-        int n = 0;
-        for (int i = 0; i < max_peaks; ++i) out[2 * i] = out[2 * i + 1] = 0.0;
-        for (int q = 1; q + 1 < n_theta; ++q) {
-            if (!(P[q] > P[q - 1] && P[q] >= P[q + 1])) continue;
-            // parabola through (q-1, q, q+1): vertex offset in cells
-            const double pm = P[q - 1], pc = P[q], pp = P[q + 1];
-            const double den = pm - 2.0 * pc + pp;
-            double dlt = 0.0;
-            if (den < 0.0) {
-                dlt = 0.5 * (pm - pp) / den;
-                dlt = dlt > 0.5 ? 0.5 : (dlt < -0.5 ? -0.5 : dlt);
-            }
-            const double th = theta[q] + dlt * (theta[q + 1] - theta[q]);
-            // insertion into the top-max_peaks list by power
-            int pos = n < max_peaks ? n : max_peaks;
-            while (pos > 0 && out[2 * (pos - 1) + 1] < pc) {
-                if (pos < max_peaks) {
-                    out[2 * pos] = out[2 * (pos - 1)];
-                    out[2 * pos + 1] = out[2 * (pos - 1) + 1];
-                }
-                --pos;
-            }
-            if (pos < max_peaks) {
-                out[2 * pos] = th;
-                out[2 * pos + 1] = pc;
-                if (n < max_peaks) ++n;
-            }
-        }
-        n_peaks[tid] = n;
+    if (part == 0 && q < n_theta) {
+        broadband[q] = ((part_c[0][ql] + part_c[1][ql]) + part_c[2][ql]) + part_c[3][ql];
+        broadband[n_theta + q] =
+            ((part_b[0][ql] + part_b[1][ql]) + part_b[2][ql]) + part_b[3][ql];
     }
+}
+
+constexpr int kMaxCandidates = 4096;
+
+// Peaks of one broadband spectrum (workgroup 0: Capon, 1: Bartlett): local
+// maxima (P[q] > P[q-1] and P[q] >= P[q+1]) are found in parallel and
+// compacted in increasing q; one thread then refines each with the
+// parabola through (q-1, q, q+1) (vertex offset clamped to +-0.5 cell) and
+// keeps the max_peaks strongest, earlier angles first on ties.
+__global__ __launch_bounds__(kThreads) void peaks_kernel(
+    const double* __restrict__ broadband, int n_theta, const double* __restrict__ theta,
+    int max_peaks, double* __restrict__ peaks /* [2][max_peaks][2] theta, power */,
+    int* __restrict__ n_peaks /* [2] */) {
+    __shared__ int cand[kMaxCandidates];
+    __shared__ int wave_count[kThreads / 64];
+    __shared__ int total;
+    const double* P = broadband + (size_t)blockIdx.x * n_theta;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    if (tid == 0) total = 0;
+    __syncthreads();
+    for (int q0 = 0; q0 < n_theta; q0 += kThreads) {
+        const int q = q0 + tid;
+        const bool peak = q >= 1 && q + 1 < n_theta && P[q] > P[q - 1] && P[q] >= P[q + 1];
+        const unsigned long long m = __ballot(peak);
+        if (lane == 0) wave_count[wave] = __popcll(m);
+        __syncthreads();
+        int base = total;
+        for (int w = 0; w < wave; ++w) base += wave_count[w];
+        if (peak) {
+            const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+            if (pos < kMaxCandidates) cand[pos] = q;
+        }
+        __syncthreads();
+        if (tid == 0)
+            for (int w = 0; w < kThreads / 64; ++w) total += wave_count[w];
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    double* out = peaks + (size_t)blockIdx.x * max_peaks * 2;
+    for (int i = 0; i < max_peaks; ++i) out[2 * i] = out[2 * i + 1] = 0.0;
+    const bool listed = total <= kMaxCandidates;
+    const int n_scan = listed ? total : n_theta;
+    int n = 0;
+    for (int i = 0; i < n_scan; ++i) {
+        const int q = listed ? cand[i] : i;
+        if (!listed && !(q >= 1 && q + 1 < n_theta && P[q] > P[q - 1] && P[q] >= P[q + 1]))
+            continue;
+        const double pm = P[q - 1], pc = P[q], pp = P[q + 1];
+        const double den = pm - 2.0 * pc + pp;
+        double dlt = 0.0;
+        if (den < 0.0) {
+            dlt = 0.5 * (pm - pp) / den;
+            dlt = dlt > 0.5 ? 0.5 : (dlt < -0.5 ? -0.5 : dlt);
+        }
+        const double th = theta[q] + dlt * (theta[q + 1] - theta[q]);
+        // insertion into the top-max_peaks list by power
+        int pos = n < max_peaks ? n : max_peaks;
+        while (pos > 0 && out[2 * (pos - 1) + 1] < pc) {
+            if (pos < max_peaks) {
+                out[2 * pos] = out[2 * (pos - 1)];
+                out[2 * pos + 1] = out[2 * (pos - 1) + 1];
+            }
+            --pos;
+        }
+        if (pos < max_peaks) {
+            out[2 * pos] = th;
+            out[2 * pos + 1] = pc;
+            if (n < max_peaks) ++n;
+        }
+    }
+    n_peaks[blockIdx.x] = n;
 }
 
 }  // namespace
 
 void launch_doa_covariance(hipStream_t stream, const uint8_t* x, size_t T, size_t F, size_t A,
                            size_t t0, size_t K, size_t M, int* rs, int* r_full) {
+    const unsigned grid = (unsigned)(8 * ceil_div(F, 8));
 #define BF_COV(A_)                                                                            \
-    hipLaunchKernelGGL(covariance_kernel<A_>, dim3((unsigned)F), dim3(kThreads), 0, stream, x, \
+    hipLaunchKernelGGL(covariance_kernel<A_>, dim3(grid), dim3(kThreads), 0, stream, x,        \
                        (int)T, (int)F, (int)t0, (int)K, (int)M, rs, r_full)
     switch (A) {
     case 32: BF_COV(32); break;
@@ -384,9 +439,11 @@ void launch_doa_fold_scan(hipStream_t stream, const int* rs, double* state, bool
 void launch_doa_estimate(hipStream_t stream, const double* capon, const double* bartlett,
                          size_t F, size_t n_theta, const double* theta, size_t max_peaks,
                          double* broadband, double* peaks, int* n_peaks) {
-    hipLaunchKernelGGL(estimate_kernel, dim3(1), dim3(kThreads), 0, stream, capon, bartlett,
-                       (int)F, (int)n_theta, theta, (int)max_peaks, broadband, peaks, n_peaks);
-    hip_check(hipGetLastError(), "estimate_kernel launch");
+    hipLaunchKernelGGL(broadband_kernel, dim3((unsigned)ceil_div(n_theta, 64)), dim3(kThreads),
+                       0, stream, capon, bartlett, (int)F, (int)n_theta, broadband);
+    hipLaunchKernelGGL(peaks_kernel, dim3(2), dim3(kThreads), 0, stream, broadband,
+                       (int)n_theta, theta, (int)max_peaks, peaks, n_peaks);
+    hip_check(hipGetLastError(), "estimate kernels launch");
 }
 
 }  // namespace bfhip

@@ -499,9 +499,21 @@ __global__ __launch_bounds__(kThreads) void tracker_mfma_kernel(
     const int item = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
     if (item >= n_items) return;
     stage_positions<A>(geo, s_pos);  // visible after tracker_windows' first barrier
+    // Items run XCD by XCD in this order: channel pairs (f, f+1), which share
+    // the 128-byte lines of every packed row, alternate within a pair's time
+    // chunks so both halves of a line are used while it sits in that L2.
     const int n_chunks = n_items / F;
-    const int f = item / n_chunks;
-    const int chunk = item - f * n_chunks;
+    const int paired = n_chunks * (F & ~1);
+    int f, chunk;
+    if (item < paired) {
+        const int pair = item / (2 * n_chunks);
+        const int r = item - pair * 2 * n_chunks;
+        f = 2 * pair + (r & 1);
+        chunk = r >> 1;
+    } else {
+        f = F - 1;
+        chunk = item - paired;
+    }
     const int wg_beam0 = blockIdx.y * kWaves * NT * 8;
     const double k = wave_number(channel_frequency(geo, f));
     const int w_begin = chunk * windows_per_chunk;
@@ -790,7 +802,12 @@ __global__ __launch_bounds__(kThreads) void serve_items_kernel(
     __syncthreads();
     const int n_long = off_l[kServeMaxSlots];
     const int n_short = off_s[kServeMaxSlots];
-    for (int item = blockIdx.x; item < n_long; item += gridDim.x) {
+    // XCD-aware order: blockIdx.x % 8 names the XCD; consecutive items
+    // (neighbouring channels) go to one XCD.
+    const int lin = gridDim.x % kXcds == 0
+                        ? (int)(blockIdx.x % kXcds) * (int)(gridDim.x / kXcds) + (int)(blockIdx.x / kXcds)
+                        : (int)blockIdx.x;
+    for (int item = lin; item < n_long; item += gridDim.x) {
         const int sidx = owner_slot(off_l, item);
         const ServeSlot* sl = slots + sidx;
         const int local = item - off_l[sidx];

@@ -47,6 +47,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-serving", action="store_true")
     p.add_argument("--serve-seconds", type=float, default=6.0)
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only run three 1 GiB device-to-device copies (known bytes for "
+                        "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
     return p.parse_args()
 
 
@@ -99,6 +102,16 @@ def main():
         sys.exit(2)
     from beamform_project_amd import api
 
+    if args.calibrate_copy:
+        n = 1 << 30
+        a, b = api.DeviceArray(n), api.DeviceArray(n)
+        a.fill(1)
+        api.device_synchronize()
+        for _ in range(3):
+            api._capi.call("bf_memcpy_dtod", b.ptr, a.ptr, n)
+        api.device_synchronize()
+        print(json.dumps({"calibration": "3 x 1 GiB device-to-device copies"}), flush=True)
+        return
     if args.workload in ("c3", "b1", "c4", "c5"):
         line = {"c3": bench_c3, "b1": bench_b1, "c4": bench_c4,
                 "c5": bench_c5}[args.workload](args)
@@ -338,7 +351,9 @@ def bench_c3(args):
          "kernel": "SPLIT_F16"},
         max_rel_err_vs_fp64=split["max_rel_err_vs_fp64"],
         roofline={"bound": "mfma", "achieved": round(ach, 2), "peak": 2500.0,
-                  "unit": "TFLOP/s", "frac": round(ach / 2500.0, 4), "traffic": None,
+                  "unit": "TFLOP/s", "frac": round(ach / 2500.0, 4),
+                  "traffic": read_profile_value("traffic_c3_tracker_split_f16.txt",
+                                                "hbm_bytes_per_launch"),
                   "kernel_avg_ms": split["kernel_avg_ms"],
                   "algorithmic_bytes": T * F_ * A + W * F_ * B * 4},
         cpu_baseline=cpu, paths=res)
@@ -468,7 +483,9 @@ def bench_c5(args):
          "n_ant": A, "n_freq": F_, "snapshots": K, "subarray": M, "n_theta": NTH},
         updates_per_s=round(1e3 / ms, 1), kernel_ms=kt, capon_peaks_deg=peaks,
         roofline={"bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
-                  "frac": round(ach / 8000.0, 4), "traffic": None,
+                  "frac": round(ach / 8000.0, 4),
+                  "traffic": read_profile_value("traffic_c5_covariance.txt",
+                                                "hbm_bytes_per_launch"),
                   "kernel": "covariance", "kernel_avg_ms": kt["covariance_ms"],
                   "algorithmic_bytes": cov_bytes},
         cpu_baseline=cpu)
@@ -567,6 +584,11 @@ def serving_run(seconds):
 def read_traffic(wl):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, or None."""
     name = "traffic_" + ("c2_split_f16" if not wl["ula"] else "c1_fp32") + ".txt"
+    return read_profile_value(name, "hbm_bytes_per_launch")
+
+
+def read_profile_value(name, key):
+    """A `key: value` figure from a committed profile summary, or None."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
@@ -578,7 +600,7 @@ def read_traffic(wl):
                 vals[k.strip()] = float(v.split()[0])
             except (ValueError, IndexError):
                 pass
-    return vals.get("hbm_bytes_per_launch")
+    return vals.get(key)
 
 
 def cpu_baseline(wl, pos, fr, dirs, seconds):
