@@ -368,8 +368,8 @@ __device__ __forceinline__ void tracker_windows(const RowSource& rs, const float
             if (q < DW_STAGE) {
                 const int tl = q / DW_TILE;
                 const int rem = q - tl * DW_TILE;
-                const int rho = rem / DW_ROW;
-                const int j = rem - rho * DW_ROW;
+                const int rho = rem & 15;  // rows fastest: 16 lanes fill
+                const int j = rem >> 4;    // 16 consecutive LDS slots
                 const int qt = q0 + tl;
                 if (qt < q_end) {
                     const int w = w_begin + qt / tpw;
@@ -390,8 +390,8 @@ __device__ __forceinline__ void tracker_windows(const RowSource& rs, const float
             if (q < DW_STAGE) {
                 const int tl = q / DW_TILE;
                 const int rem = q - tl * DW_TILE;
-                const int rho = rem / DW_ROW;
-                const int j = rem - rho * DW_ROW;
+                const int rho = rem & 15;  // rows fastest: 16 lanes fill
+                const int j = rem >> 4;    // 16 consecutive LDS slots
                 const int g = j / (A / 16);
                 const int s = j - g * (A / 16);
                 unsigned h2[4];

@@ -164,8 +164,8 @@ __global__ __launch_bounds__(WV * kWaveSize) void mfma_fused_kernel(
             if (q < DW_STAGE) {
                 const int tl = q / DW_TILE;
                 const int rem = q - tl * DW_TILE;
-                const int rho = rem / DW_ROW;
-                const int j = rem - rho * DW_ROW;
+                const int rho = rem & 15;  // rows fastest: 16 lanes fill
+                const int j = rem >> 4;    // 16 consecutive LDS slots
                 const int t = tile_time<MODE>(stage_tile0 + tl, rho >> 2, rho & 3, S);
                 if (t < T)
                     v = *reinterpret_cast<const unsigned*>(xf + (size_t)t * row_stride + 4 * j);
@@ -180,8 +180,8 @@ __global__ __launch_bounds__(WV * kWaveSize) void mfma_fused_kernel(
             if (q < DW_STAGE) {
                 const int tl = q / DW_TILE;
                 const int rem = q - tl * DW_TILE;
-                const int rho = rem / DW_ROW;
-                const int j = rem - rho * DW_ROW;
+                const int rho = rem & 15;  // rows fastest: 16 lanes fill
+                const int j = rem >> 4;    // 16 consecutive LDS slots
                 const int g = j / (A / 16);
                 const int s = j - g * (A / 16);
                 unsigned h2[4];
