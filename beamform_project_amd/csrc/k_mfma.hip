@@ -27,20 +27,45 @@ std::map<TuneKey, MfmaConfig>& tune_cache() {
     return cache;
 }
 
-// Beams per wave: as many 8-beam N-tiles as the beam count needs, capped so
-// that the split weights take at most 128 VGPRs per lane.  Two LDS stages
-// and chunks sized for 1024 resident workgroups measured fastest on MI355X
-// for the full-shard problem (A=64, B=128, S=10: 1.21 ms against 1.39 ms
-// with one stage); at A=256 the second stage would spill registers.
-MfmaConfig default_config(int A, int B) {
-    const int max_nt = A <= 64 ? 4 : (A == 128 ? 2 : 1);
-    const int need = (B + 31) / 32;
+// Untuned shape per antenna count, beam count and precision: the fastest
+// measured on MI355X for representative problems (F = 336, S = 10 and 320,
+// 1-3 ms launches).  With many beams at A = 64 the split kernel runs two
+// N-tiles on each of eight waves (126 registers: two workgroups per CU)
+// and the single-FP16 kernel four N-tiles on four; fewer beams use fewer
+// N-tiles.  A single LDS stage wins from 64 antennas up, two stages at
+// 32 and for the split kernel at 128.
+MfmaConfig default_config(int A, int B, bool split) {
+    const int need = (B + 7) / 8;  // 8-beam N-tiles
     MfmaConfig c;
     c.waves = 4;
-    c.nt = need <= 1 ? 1 : (need <= 2 ? std::min(2, max_nt) : max_nt);
-    if (A == 256) c.nt = 1;
-    c.db = A == 256 ? 0 : 1;
-    c.resident = A == 256 ? 512 : 1024;
+    c.db = 0;
+    c.resident = 512;
+    if (A == 32) {
+        c.nt = need <= 4 ? 1 : 2;
+        c.waves = need <= 8 ? 4 : 8;
+        c.db = 1;
+        c.resident = 256;
+    } else if (A == 64) {
+        if (need <= 2) {
+            c.nt = 1;
+        } else if (need <= 8) {
+            c.nt = 2;
+        } else if (split) {
+            c.nt = 2;
+            c.waves = 8;
+            c.resident = 256;
+        } else {
+            c.nt = 4;
+            c.resident = 256;
+        }
+    } else if (A == 128) {
+        c.nt = need <= 4 ? 1 : 2;
+        c.db = split ? 1 : 0;
+        c.resident = 256;
+    } else {
+        c.nt = 1;
+        c.resident = 1024;
+    }
     return c;
 }
 
@@ -80,7 +105,7 @@ bool mfma_supported(size_t A, size_t B, size_t S) {
 void launch_mfma_fused(hipStream_t stream, bool split, const uint8_t* x, const float* w,
                        float* out, size_t T, size_t F, size_t A, size_t B, size_t S) {
     check_problem(T, F, A, B, S);
-    MfmaConfig cfg = default_config((int)A, (int)B);
+    MfmaConfig cfg = default_config((int)A, (int)B, split);
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
         auto it = tune_cache().find(TuneKey((int)T, (int)F, (int)A, (int)B, (int)S, split));
@@ -103,6 +128,10 @@ std::vector<MfmaTuneResult> autotune_mfma_fused(hipStream_t stream, bool split,
     hip_check(hipEventCreate(&e1), "hipEventCreate");
     std::vector<MfmaTuneResult> results;
     try {
+        // Bring the clocks up before the first candidate is timed.
+        const MfmaConfig warm = default_config((int)A, (int)B, split);
+        for (int r = 0; r < 3; ++r) launch_cfg(stream, L, (int)A, warm);
+        hip_check(hipGetLastError(), "mfma_fused_kernel launch");
         for (const auto& sh : shapes_for((int)A)) {
             for (int db = 0; db < 2; ++db) {
                 for (int resident : {256, 512, 1024}) {

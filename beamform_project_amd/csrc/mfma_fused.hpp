@@ -18,10 +18,15 @@
 // are loaded once per workgroup with coalesced dword loads, converted to
 // FP16 pairs (byte permute, xor, one packed subtraction) and written to LDS
 // in the MFMA A-operand layout, so each lane reads its operand with one
-// 16-byte LDS load per k-step.  The next stage's global loads are in flight
-// while the current stage is computed.  With DB the LDS holds two stages:
-// the next stage is written while other waves still compute the current
-// one, so a stage costs one barrier instead of two.
+// 16-byte LDS load per k-step.  Every staging address is a per-thread
+// constant plus a wave-uniform (scalar) stage offset.  The next stage's
+// global loads are in flight while the current stage is computed.  With
+// DB the LDS holds two stages and the conversion and stores of the next
+// stage, and the loads of the one after, are spread over the tiles of the
+// current stage, inside the same (fully unrolled) block as its MFMAs, so
+// this VALU work issues between MFMAs; a stage costs one barrier.  The
+// window bookkeeping of a tile is static in the unrolled stage, and the
+// two lanes of a re/im pair split each window's stores.
 //
 // Output layout of a 16x16 MFMA: lane l holds column l%16 for rows
 // 4*(l/16)+r, r = 0..3.  Tile rows are mapped to time so that row group
@@ -33,6 +38,8 @@
 // Re and im of a beam sit in adjacent lanes; a quad-permute DPP adds the
 // two squared sums.
 #pragma once
+
+#include <type_traits>
 
 #include "bf_kernels.hpp"
 
@@ -62,15 +69,6 @@ __device__ __forceinline__ float dpp_swap_pairs(float v) {
     // quad_perm [1,0,3,2]: lane i <- lane i^1.
     return __builtin_bit_cast(
         float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true));
-}
-
-// Time of row (G, r) of global tile i.
-template <int MODE>
-__device__ __forceinline__ int tile_time(int i, int G, int r, int S) {
-    if (MODE == 0) return 16 * i + 4 * G + r;
-    if (MODE == 1) return 80 * (i / 5) + 20 * G + 4 * (i % 5) + r;
-    const int tps = S >> 2;
-    return 4 * S * (i / tps) + S * G + 4 * (i % tps) + r;
 }
 
 // Loads the B operand (weights) of one wave into registers.
@@ -112,19 +110,48 @@ __device__ __forceinline__ void load_weights(const float2* __restrict__ w, int F
     }
 }
 
+// Staging map of one stage: dword k of a thread is q = tid + k*kThreads of
+// the stage, i.e. tile tl0 + tld(k) and in-tile dword rem0 + remd(k), where
+// tl0, rem0 depend on the thread only and tld, remd are compile-time (both
+// kThreads and the tile's dword count 4A are powers of two).
+template <int A, int kThreads>
+struct StageMap {
+    static constexpr int DW_TILE = 4 * A;
+    static constexpr int TILE_BYTES = (A / 16) * kWaveSize * 16;
+    static constexpr int tld(int k) {
+        return kThreads >= DW_TILE ? k * (kThreads / DW_TILE) : (k * kThreads) / DW_TILE;
+    }
+    static constexpr int remd(int k) {
+        return kThreads >= DW_TILE ? 0 : (k * kThreads) % DW_TILE;
+    }
+    // dword j of a row sits in k-step j % (A/16) of lane group j / (A/16);
+    // remd is a multiple of 256, so it moves the lane group only.
+    static constexpr int lds_delta(int k) {
+        return tld(k) * TILE_BYTES + 16 * 16 * ((remd(k) >> 4) / (A / 16));
+    }
+    static constexpr int byte_delta(int k) { return 4 * (remd(k) >> 4); }
+};
+
+// At least two waves per SIMD (at most 256 registers per lane): the
+// unrolled stage otherwise keeps several tiles' accumulators in flight.
 template <int A, int NT, int WV, bool SPLIT, int MODE, int ST, bool DB>
-__global__ __launch_bounds__(WV * kWaveSize) void mfma_fused_kernel(
+__global__ __launch_bounds__(WV * kWaveSize) __attribute__((amdgpu_waves_per_eu(2)))
+void mfma_fused_kernel(
     const uint8_t* __restrict__ x, const float2* __restrict__ w, float* __restrict__ out,
     int T, int F, int B, int S, int W, int tiles_total, int tiles_per_chunk, int n_items,
     int per_xcd) {
     constexpr int kThreads = WV * kWaveSize;
     constexpr int KS = A / 16;                      // k-steps of 32 (16 antennas x re/im)
-    constexpr int TILE_BYTES = KS * kWaveSize * 16;  // LDS bytes per 16-row tile
+    using Map = StageMap<A, kThreads>;
+    constexpr int TILE_BYTES = Map::TILE_BYTES;     // LDS bytes per 16-row tile
     constexpr int STAGE_BYTES = ST * TILE_BYTES;
-    constexpr int DW_ROW = A / 4;                    // packed dwords per row
-    constexpr int DW_TILE = 16 * DW_ROW;
+    constexpr int DW_TILE = Map::DW_TILE;           // packed dwords per tile
     constexpr int DW_STAGE = ST * DW_TILE;
     constexpr int DW_THREAD = (DW_STAGE + kThreads - 1) / kThreads;
+    constexpr bool ALL_VALID = DW_STAGE % kThreads == 0;
+    // MODE 1 stages hold whole super-tiles of 5 tiles
+    static_assert(MODE != 1 || ST % 5 == 0, "S = 10 stages are whole super-tiles");
+    static_assert(kThreads % DW_TILE == 0 || DW_TILE % kThreads == 0, "power-of-two staging");
     __shared__ __attribute__((aligned(16))) unsigned char lds[(DB ? 2 : 1) * STAGE_BYTES];
 
     const int tid = threadIdx.x;
@@ -152,44 +179,73 @@ __global__ __launch_bounds__(WV * kWaveSize) void mfma_fused_kernel(
     int tile_end = tile_begin + tiles_per_chunk;
     tile_end = tile_end < tiles_total ? tile_end : tiles_total;
     const size_t row_stride = (size_t)F * A;
-    const uint8_t* xf = x + (size_t)f * A;
+    const int tps = MODE == 2 ? (S >> 2) : 5;           // tiles per super-tile
+    const int pitch = MODE == 0 ? 4 : (MODE == 1 ? 20 : S);  // spectra between row groups
 
-    // Stage loader: dword q of the stage -> (tile, row, dword in row).
-    unsigned pre[DW_THREAD];
-    auto load_stage = [&](int stage_tile0) {
-#pragma unroll
-        for (int k = 0; k < DW_THREAD; ++k) {
-            const int q = tid + k * kThreads;
-            unsigned v = 0u;
-            if (q < DW_STAGE) {
-                const int tl = q / DW_TILE;
-                const int rem = q - tl * DW_TILE;
-                const int rho = rem & 15;  // rows fastest: 16 lanes fill
-                const int j = rem >> 4;    // 16 consecutive LDS slots
-                const int t = tile_time<MODE>(stage_tile0 + tl, rho >> 2, rho & 3, S);
-                if (t < T)
-                    v = *reinterpret_cast<const unsigned*>(xf + (size_t)t * row_stride + 4 * j);
-            }
-            pre[k] = v;
-        }
+    // Per-thread staging constants (rows fastest within 16 lanes, so the
+    // 16-byte LDS slots s*64 + row + 16g written by 16 consecutive lanes are
+    // contiguous and each global load reads 16 contiguous bytes per row).
+    // tl0 is uniform per wave (a tile's dwords span whole waves)
+    const int tl0 = kThreads >= DW_TILE
+                        ? __builtin_amdgcn_readfirstlane(wave * kWaveSize / DW_TILE)
+                        : 0;
+    const int rem0 = kThreads >= DW_TILE ? tid % DW_TILE : tid;
+    const int rho = rem0 & 15;
+    const int j0 = rem0 >> 4;
+    // spectrum of this thread's row within its tile's row group pattern
+    const int prow = (rho >> 2) * pitch + (rho & 3);
+    const uint8_t* xs = x + (size_t)f * A + (size_t)prow * row_stride + 4 * j0;
+    unsigned char* const lds_mine =
+        lds + tl0 * TILE_BYTES + ((j0 % KS) * kWaveSize + rho + 16 * (j0 / KS)) * 16;
+
+    // First spectrum of the stage starting at tile st0, and offset of tile
+    // tl of a stage from it (row group 0, row 0).
+    auto stage_time = [&](int st0) -> int {
+        if (MODE == 0 || MODE == 1) return 16 * st0;  // MODE 1: 80 per super-tile
+        return 4 * S * (st0 / tps) + 4 * (st0 % tps);
     };
-    auto store_stage = [&](unsigned char* base) {
+    auto tile_rows = [&](int tl) -> int {
+        if (MODE == 0) return 16 * tl;
+        if (MODE == 1) return ST == 5 ? 4 * tl : 80 * (tl / 5) + 4 * (tl % 5);
+        return 4 * tl;
+    };
+    // Whether every row of the stage at st0 lies before T.
+    auto stage_full = [&](int st0) -> bool {
+        return stage_time(st0) + tile_rows(ST - 1) + 3 * pitch + 3 < T;
+    };
+    auto dword_valid = [&](int k) -> bool {
+        return ALL_VALID || tl0 + Map::tld(k) < ST;
+    };
+    unsigned pre[DW_THREAD];
+    // Global load of dword k of the stage whose first spectrum is t_stage;
+    // GUARD: rows at or after t_lim read as zero.
+    auto load_dword = [&](auto guard, int k, int t_stage, int t_lim) {
+        constexpr bool GUARD = decltype(guard)::value;
+        const int rel = t_stage + tile_rows(tl0 + Map::tld(k));
+        const uint8_t* p = xs + (size_t)rel * row_stride + Map::byte_delta(k);
+        unsigned v = 0u;
+        if (GUARD) {
+            if (dword_valid(k) && prow + rel < t_lim) v = *reinterpret_cast<const unsigned*>(p);
+        } else if (dword_valid(k)) {
+            v = *reinterpret_cast<const unsigned*>(p);
+        }
+        pre[k] = v;
+    };
+    auto store_dword = [&](int k, unsigned char* buf) {
+        if (!dword_valid(k)) return;
+        unsigned h2[4];
+        bf_int4x4_to_half2x4(pre[k], h2);
+        bf_uint4v u = {h2[0], h2[1], h2[2], h2[3]};
+        *reinterpret_cast<bf_uint4v*>(buf + (lds_mine - lds) + Map::lds_delta(k)) = u;
+    };
+    auto load_stage = [&](int st0) {
+        const int ts = stage_time(st0);
+        if (stage_full(st0)) {
 #pragma unroll
-        for (int k = 0; k < DW_THREAD; ++k) {
-            const int q = tid + k * kThreads;
-            if (q < DW_STAGE) {
-                const int tl = q / DW_TILE;
-                const int rem = q - tl * DW_TILE;
-                const int rho = rem & 15;  // rows fastest: 16 lanes fill
-                const int j = rem >> 4;    // 16 consecutive LDS slots
-                const int g = j / (A / 16);
-                const int s = j - g * (A / 16);
-                unsigned h2[4];
-                bf_int4x4_to_half2x4(pre[k], h2);
-                bf_uint4v u = {h2[0], h2[1], h2[2], h2[3]};
-                *reinterpret_cast<bf_uint4v*>(base + tl * TILE_BYTES +
-                                              (s * kWaveSize + rho + 16 * g) * 16) = u;
-            }
+            for (int k = 0; k < DW_THREAD; ++k) load_dword(std::false_type{}, k, ts, T);
+        } else {
+#pragma unroll
+            for (int k = 0; k < DW_THREAD; ++k) load_dword(std::true_type{}, k, ts, T);
         }
     };
 
@@ -198,121 +254,216 @@ __global__ __launch_bounds__(WV * kWaveSize) void mfma_fused_kernel(
     for (int nt = 0; nt < NT; ++nt) win[nt][0] = win[nt][1] = 0.0f;
     const int col = lane & 15;
     const int G = lane >> 4;
-    const int tps = MODE == 2 ? (S >> 2) : 5;
+    const int odd = col & 1;
+    // Output pointer of this lane at window (or spectrum) group 0.  Lanes
+    // 2c and 2c+1 hold the same power after the pair swap, so they split
+    // the stores: MODE 1 by window of the pair (q = odd), MODE 2 by N-tile
+    // (nt + odd), MODE 0 by row (r = odd + 2m).
+    const int bl = beam0 + (col >> 1);
+    float* const out_lane =
+        MODE == 1 ? out + ((size_t)(2 * G + odd) * F + f) * B + bl
+        : MODE == 2 ? out + ((size_t)G * F + f) * B + bl + (NT >= 2 ? 8 * odd : 0)
+                    : out + ((size_t)(4 * G + odd) * F + f) * B + bl;
+    const size_t group_stride = (size_t)F * B;  // floats between consecutive windows
 
-    // MFMAs and epilogue of one stage held in LDS at `base`.
-    auto compute_stage = [&](const unsigned char* base, int st0) {
-        // One tile ahead: the A operand of tile tl+1 is read from LDS while
-        // the MFMAs of tile tl run.
-        bf_half8 av[KS];
+    // Powers and window sums of tile tl (global tile i) from acc.
+    auto epilogue = [&](int tl, int i, const bf_float4 (&acc)[NT]) {
+        if (MODE == 0) {
+            float* const o = out_lane + (size_t)(16 * i) * group_stride;
+            const int t0 = 16 * i + 4 * G + odd;
 #pragma unroll
-        for (int s = 0; s < KS; ++s)
-            av[s] = *reinterpret_cast<const bf_half8*>(base + (s * kWaveSize + lane) * 16);
-#pragma unroll 1
-        for (int tl = 0; tl < ST; ++tl) {
-            const int i = st0 + tl;
-            bf_half8 an[KS];
-            if (tl + 1 < ST) {
+            for (int nt = 0; nt < NT; ++nt) {
+                float p[4];
 #pragma unroll
-                for (int s = 0; s < KS; ++s)
-                    an[s] = *reinterpret_cast<const bf_half8*>(
-                        base + (tl + 1) * TILE_BYTES + (s * kWaveSize + lane) * 16);
-            }
-            bf_float4 acc[NT];
+                for (int r = 0; r < 4; ++r) {
+                    const float sq = acc[nt][r] * acc[nt][r];
+                    p[r] = sq + dpp_swap_pairs(sq);
+                }
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[nt] = bf_float4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int s = 0; s < KS; ++s) {
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[s], bhi[nt][s], acc[nt],
-                                                                     0, 0, 0);
-                if (SPLIT) {
-#pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
-                        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[s], blo[nt][s],
-                                                                         acc[nt], 0, 0, 0);
+                for (int m = 0; m < 2; ++m) {
+                    const float v = odd ? p[2 * m + 1] : p[2 * m];
+                    if (bl + nt * 8 < B && t0 + 2 * m < T)
+                        o[(size_t)(2 * m) * group_stride + nt * 8] = v;
                 }
             }
+            return;
+        }
+        if (MODE == 1 && tl % 5 == 2) {
+            // spectra 8, 9 close window 0; 10, 11 open window 1
 #pragma unroll
-            for (int s = 0; s < KS; ++s) av[s] = an[s];
-            if (MODE == 0) {
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    const int b = beam0 + nt * 8 + (col >> 1);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const float sq = acc[nt][r] * acc[nt][r];
-                        const float p = sq + dpp_swap_pairs(sq);
-                        const int t = tile_time<0>(i, G, r, S);
-                        if ((col & 1) == 0 && b < B && t < T)
-                            out[((size_t)t * F + f) * B + b] = p;
-                    }
-                }
-                continue;
+            for (int nt = 0; nt < NT; ++nt) {
+                win[nt][0] = fmaf(acc[nt][0], acc[nt][0], win[nt][0]);
+                win[nt][0] = fmaf(acc[nt][1], acc[nt][1], win[nt][0]);
+                win[nt][1] = fmaf(acc[nt][2], acc[nt][2], win[nt][1]);
+                win[nt][1] = fmaf(acc[nt][3], acc[nt][3], win[nt][1]);
             }
-            const int ii = MODE == 1 ? tl : (i % tps);  // tile within super-tile
-            if (MODE == 1 && tl == 2) {
-                // spectra 8, 9 close window 0; 10, 11 open window 1
+            return;
+        }
+        const int q = (MODE == 1 && tl % 5 > 2) ? 1 : 0;  // static once the tile loop is unrolled
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    win[nt][0] = fmaf(acc[nt][0], acc[nt][0], win[nt][0]);
-                    win[nt][0] = fmaf(acc[nt][1], acc[nt][1], win[nt][0]);
-                    win[nt][1] = fmaf(acc[nt][2], acc[nt][2], win[nt][1]);
-                    win[nt][1] = fmaf(acc[nt][3], acc[nt][3], win[nt][1]);
-                }
+        for (int nt = 0; nt < NT; ++nt) {
+            float a = acc[nt][0] * acc[nt][0];
+#pragma unroll
+            for (int r = 1; r < 4; ++r) a = fmaf(acc[nt][r], acc[nt][r], a);
+            if (q == 0)
+                win[nt][0] += a;
+            else
+                win[nt][1] += a;
+        }
+    };
+    // Stores and resets the windows of the super-tile (MODE 1) or stage
+    // (MODE 2) that starts at tile i0.
+    auto close_windows = [&](int i0) {
+        if (MODE == 1) {
+            const int super = i0 / 5;
+            float* const o = out_lane + (size_t)(8 * super) * group_stride;
+            const bool ok = 8 * super + 2 * G + odd < W;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const float p0 = win[nt][0] + dpp_swap_pairs(win[nt][0]);
+                const float p1 = win[nt][1] + dpp_swap_pairs(win[nt][1]);
+                if (ok && bl + nt * 8 < B) o[nt * 8] = odd ? p1 : p0;
+                win[nt][0] = win[nt][1] = 0.0f;
+            }
+        } else if (MODE == 2) {
+            const int super = i0 / tps;
+            float* const o = out_lane + (size_t)(4 * super) * group_stride;
+            const bool ok = 4 * super + G < W;
+            float p[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                p[nt] = win[nt][0] + dpp_swap_pairs(win[nt][0]);
+                win[nt][0] = 0.0f;
+            }
+            if (NT == 1) {
+                if (ok && !odd && bl < B) o[0] = p[0];
             } else {
-                const int q = (MODE == 1 && tl > 2) ? 1 : 0;  // wave-uniform
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    float a = 0.0f;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) a = fmaf(acc[nt][r], acc[nt][r], a);
-                    // both updates, no dynamic index (x + 0 == x for these sums)
-                    win[nt][0] += q ? 0.0f : a;
-                    win[nt][1] += q ? a : 0.0f;
-                }
-            }
-            if (ii == tps - 1) {
-                constexpr int kWinPerGroup = MODE == 1 ? 2 : 1;
-                const int super = i / tps;
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    const int b = beam0 + nt * 8 + (col >> 1);
-#pragma unroll
-                    for (int q = 0; q < kWinPerGroup; ++q) {
-                        const float p = win[nt][q] + dpp_swap_pairs(win[nt][q]);
-                        const int wi = super * 4 * kWinPerGroup + G * kWinPerGroup + q;
-                        if ((col & 1) == 0 && b < B && wi < W)
-                            out[((size_t)wi * F + f) * B + b] = p;
-                        win[nt][q] = 0.0f;
-                    }
-                }
+                for (int nt = 0; nt < NT; nt += 2)
+                    if (ok && bl + (nt + odd) * 8 < B) o[nt * 8] = odd ? p[nt + 1] : p[nt];
             }
         }
     };
+    // MODE 2: whether the stage at st0 ends a super-tile (MODE 1 closes
+    // its windows inside the stage, after every fifth tile).
+    auto closes_window = [&](int st0) -> bool {
+        return MODE == 2 && (st0 % tps) + ST == tps;
+    };
 
-    if (tile_begin < tile_end) load_stage(tile_begin);
+    // MFMAs and epilogues of the stage held at `cur`; with DB, the dwords
+    // of the next stage (in pre) are converted into `nxt` and the loads of
+    // the stage after it are issued, spread over the tiles so that this
+    // VALU and memory work interleaves with the MFMAs.
+    auto stage = [&](auto mf, auto guard, const unsigned char* cur, unsigned char* nxt, int st0,
+                     int t_load, int t_lim) {
+        constexpr bool MF = decltype(mf)::value;
+        // Up to 64 antennas the next tile's A operands are read while this
+        // tile's MFMAs run; wider tiles read them per k-step (a second
+        // tile of operands would not fit in registers).
+        constexpr bool PREFETCH = KS <= 4;
+        bf_half8 av[KS];
+        if (MF && PREFETCH) {
+#pragma unroll
+            for (int s = 0; s < KS; ++s)
+                av[s] = *reinterpret_cast<const bf_half8*>(cur + (s * kWaveSize + lane) * 16);
+        }
+#pragma unroll
+        for (int tl = 0; tl < ST; ++tl) {
+            bf_half8 an[KS];
+            bf_float4 acc[NT];
+            if (MF) {
+                if (PREFETCH && tl + 1 < ST) {
+#pragma unroll
+                    for (int s = 0; s < KS; ++s)
+                        an[s] = *reinterpret_cast<const bf_half8*>(
+                            cur + (tl + 1) * TILE_BYTES + (s * kWaveSize + lane) * 16);
+                }
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) acc[nt] = bf_float4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int s = 0; s < KS; ++s) {
+                    const bf_half8 a_s =
+                        PREFETCH ? av[s]
+                                 : *reinterpret_cast<const bf_half8*>(
+                                       cur + tl * TILE_BYTES + (s * kWaveSize + lane) * 16);
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt)
+                        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_s, bhi[nt][s],
+                                                                         acc[nt], 0, 0, 0);
+                    if (SPLIT) {
+#pragma unroll
+                        for (int nt = 0; nt < NT; ++nt)
+                            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_s, blo[nt][s],
+                                                                             acc[nt], 0, 0, 0);
+                    }
+                }
+            }
+            if (DB) {
+#pragma unroll
+                for (int k = tl * DW_THREAD / ST; k < (tl + 1) * DW_THREAD / ST; ++k) {
+                    store_dword(k, nxt);
+                    load_dword(guard, k, t_load, t_lim);
+                }
+            }
+            if (MF) {
+                epilogue(tl, st0 + tl, acc);
+                if (MODE == 1 && tl % 5 == 4) close_windows(st0 + tl - 4);
+                if (PREFETCH && tl + 1 < ST) {
+#pragma unroll
+                    for (int s = 0; s < KS; ++s) av[s] = an[s];
+                }
+            }
+            // Wide tiles: keep the scheduler from hoisting the next tile's
+            // operand reads into this one (registers).
+            if (!PREFETCH) __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+
     if (DB) {
-        if (tile_begin < tile_end) store_stage(lds);
-        __syncthreads();
+        if (tile_begin < tile_end) {
+            load_stage(tile_begin);
+#pragma unroll
+            for (int k = 0; k < DW_THREAD; ++k) store_dword(k, lds);
+        }
         if (tile_begin + ST < tile_end) load_stage(tile_begin + ST);
+        __syncthreads();
         int buf = 0;
         for (int st0 = tile_begin; st0 < tile_end; st0 += ST) {
-            if (has_beams) compute_stage(lds + buf * STAGE_BYTES, st0);
+            const unsigned char* cur = lds + buf * STAGE_BYTES;
+            unsigned char* nxt = lds + (buf ^ 1) * STAGE_BYTES;
             // The other buffer was last read before the previous barrier.
-            if (st0 + ST < tile_end) store_stage(lds + (buf ^ 1) * STAGE_BYTES);
+            // Loads of the stage after next: unguarded when it exists and
+            // lies before T; otherwise guarded (t_lim 0 when it does not exist).
+            const int st2 = st0 + 2 * ST;
+            const int t_load = stage_time(st2);
+            if (st2 < tile_end && stage_full(st2)) {
+                if (has_beams)
+                    stage(std::true_type{}, std::false_type{}, cur, nxt, st0, t_load, T);
+                else
+                    stage(std::false_type{}, std::false_type{}, cur, nxt, st0, t_load, T);
+            } else {
+                const int t_lim = st2 < tile_end ? T : 0;
+                if (has_beams)
+                    stage(std::true_type{}, std::true_type{}, cur, nxt, st0, t_load, t_lim);
+                else
+                    stage(std::false_type{}, std::true_type{}, cur, nxt, st0, t_load, t_lim);
+            }
+            if (has_beams && closes_window(st0)) close_windows(st0);
             __syncthreads();
-            if (st0 + 2 * ST < tile_end) load_stage(st0 + 2 * ST);
             buf ^= 1;
         }
     } else {
+        if (tile_begin < tile_end) load_stage(tile_begin);
         for (int st0 = tile_begin; st0 < tile_end; st0 += ST) {
             __syncthreads();  // previous stage fully consumed
-            store_stage(lds);
-            __syncthreads();  // stage visible to all waves
+#pragma unroll
+            for (int k = 0; k < DW_THREAD; ++k) store_dword(k, lds);
             if (st0 + ST < tile_end) load_stage(st0 + ST);  // in flight during compute
-            if (has_beams) compute_stage(lds, st0);
+            __syncthreads();  // stage visible to all waves
+            if (has_beams) {
+                stage(std::true_type{}, std::false_type{}, lds, lds, st0, 0, 0);
+                if (closes_window(st0)) close_windows(st0);
+            }
         }
     }
 }
