@@ -6,13 +6,14 @@
 // w[b][f][a] = (float)cos(phi), (float)sin(phi) with the phase
 // phi = (2 pi f / c) * (p_a . d) evaluated in FP64 (src/weights.cpp:37-55),
 // then contracts like the fixed grid (src/cpu_beamformer.cpp:56-78).
-// Weights are generated inside the kernels, per window, never stored:
-// cos/sin are evaluated in FP64 as the reference does, except that the
-// MFMA kernels, for windows shorter than 64 spectra (where weight
-// generation would dominate), evaluate every 16th window from the FP64
-// phase reduced to [-pi, pi] in FP32 and rotate it by the small phase
-// change to the windows in between (a few float ulps, below the FP16
-// split's rounding).
+// Weights are generated inside the kernels, per window, never stored.  The
+// FP32 kernel evaluates cos/sin in FP64 as the reference does.  The MFMA
+// kernels form the same FP64 phase, reduce it to [-pi, pi] in FP64 and
+// evaluate it in FP32 with a first-order correction (about one float ulp
+// from (float)cos(phi), below the FP16 split's own rounding, at a fraction
+// of an FP64 sincos); for windows shorter than 64 spectra (where weight
+// generation would dominate) they do so for every 16th window and rotate
+// it by the small phase change to the windows in between.
 //
 // This translation unit is compiled without FMA contraction so that the
 // direction (float) and phase (double) arithmetic rounds exactly like the
@@ -32,6 +33,7 @@
 // of a lane column with two cross-lane adds at the window's last tile.
 #include "bf_kernels.hpp"
 #include "bf_serve.hpp"
+#include "mfma_stage.hpp"
 #include "bf_tracker.hpp"
 
 namespace bfhip {
@@ -122,7 +124,7 @@ __device__ __forceinline__ float channel_frequency(const Geom& geo, int f) {
     return geo.f_start + step;
 }
 
-// cos/sin of phi = k * (p . d) for short windows: the phase is formed and
+// cos/sin of phi = k * (p . d) for the MFMA kernels: the phase is formed and
 // reduced to [-pi, pi] in FP64, then evaluated in FP32 with a first-order
 // correction for the rounding of the reduced argument, so the weights
 // differ from (float)cos(phi), (float)sin(phi) by about one float ulp --
@@ -141,15 +143,15 @@ __device__ __forceinline__ float2 steering_weight_reduced(const float p[3], cons
     return make_float2(fmaf(-rl, sn, cs), fmaf(rl, cs, sn));
 }
 
-// Windows of at least this many spectra take the weights from an FP64
-// sincos, as the reference does.  Shorter windows, where weight generation
-// would dominate, use anchored weights: every kAnchorWindows-th window (by
+// Windows of at least this many spectra evaluate every window's weights
+// from its own phase (steering_weight_reduced).  Shorter windows, where
+// weight generation would dominate, use anchored weights: every kAnchorWindows-th window (by
 // window index) is an anchor evaluated with steering_weight_reduced, and
 // window w rotates the anchor's phasor by the phase change
 // delta = k * (p . (d_w - d_anchor)), formed in FP32 from the small
 // direction difference.  Each window's weights depend only on w and its
 // anchor, never on how windows are grouped into launches.
-constexpr int kExactWeightMinSpectra = 64;
+constexpr int kDirectWeightMinSpectra = 64;
 constexpr int kAnchorWindows = 16;
 
 // Anchor phasors of this lane's own weights and the anchor's directions.
@@ -200,21 +202,21 @@ __device__ __forceinline__ unsigned dpp_swap_pairs_u(unsigned v) {
 // evaluates two of every four (j = 2*comp_im + jj), splits them into hi/lo
 // FP16 pairs and takes the partner's two pairs by DPP; a Re column then
 // holds (w_re, -w_im) (sign flip of the high half) and an Im column
-// (w_im, w_re) (halves swapped).  Without ANCHORED (or with `exact`) every
-// weight comes from an FP64 sincos; with ANCHORED the anchor window's
+// (w_im, w_re) (halves swapped).  Without ANCHORED (or with `direct`) every
+// weight comes from steering_weight_reduced; with ANCHORED the anchor window's
 // phasors are kept in `anc` and rotated to window w.  Positions come from
 // the LDS copy s_pos.
 template <int A, int NT, bool SPLIT, int DIRSRC, bool ANCHORED>
 __device__ __forceinline__ void window_operands(const DirArgs& da, const float* s_pos,
                                                 double k, int w, int beam0, int lane,
-                                                bool exact, WeightAnchor<A, NT>& anc,
+                                                bool direct, WeightAnchor<A, NT>& anc,
                                                 bf_half8 (&bhi)[NT][A / 16],
                                                 bf_half8 (&blo)[NT][A / 16]) {
     constexpr int KS = A / 16;
     const int c = lane & 15;
     const int g = lane >> 4;
     const int comp_im = c & 1;
-    const bool use_anchor = ANCHORED && !exact;
+    const bool use_anchor = ANCHORED && !direct;
     const int wa = w - w % kAnchorWindows;
     if (use_anchor && anc.window != wa) {
 #pragma unroll
@@ -271,7 +273,7 @@ __device__ __forceinline__ void window_operands(const DirArgs& da, const float* 
                     }
                 } else {
                     v = make_float2(0.0f, 0.0f);
-                    if (live) v = steering_weight(p, d, k);
+                    if (live) v = steering_weight_reduced(p, d, k);
                 }
                 split_pack(v, SPLIT, ph[jj], pl[jj]);
             }
@@ -323,26 +325,39 @@ struct OutTarget {
     int f;
 };
 
+// A tile of a tracker call: its first spectrum (absolute) and how many of
+// its 16 rows hold spectra of the window (0 past the call's tiles).
+struct TileRef {
+    long t;
+    int rows;
+};
+
 // Windows [w_begin, w_end) of one channel for the beams wg_beam0 ..
 // wg_beam0 + kWaves*NT*8 - 1.  Entered by the whole workgroup (kThreads
 // threads): stages of ST window-aligned tiles of 16 spectra go through LDS
-// in the A-operand layout, the next stage is held in registers meanwhile,
-// and the B operand (a window's weights) is regenerated at each window's
-// first tile.  Each wave owns one block of NT*8 beams; when a window fits
-// one tile and fewer than kWaves blocks hold beams, the idle waves share
-// the blocks and the tiles of each stage are dealt out among a block's
-// waves, so every wave generates weights.
+// in the A-operand layout (two stage buffers, one barrier per stage).  The
+// packed rows of stage s + 2 are loaded at the start of stage s and those
+// of stage s + 1, loaded a stage earlier, are converted into the other
+// buffer at its end, so every load has two stages of compute to land.
+// Each thread's rows sit at a per-thread offset from a wave-uniform tile
+// base; a stage whose tiles are all full and do not wrap the ring loads
+// without guards.  The B operand (a window's weights) is regenerated at
+// each window's first tile.  Each wave owns one block of NT*8 beams; when a
+// window fits one tile and fewer than kWaves blocks hold beams, the idle
+// waves share the blocks and the tiles of each stage are dealt out among a
+// block's waves, so every wave generates weights.
 template <int A, int NT, bool SPLIT, bool INTEGRATE, int DIRSRC, int ST, bool ANCHORED>
 __device__ __forceinline__ void tracker_windows(const RowSource& rs, const float* s_pos,
                                                 const DirArgs& da, const OutTarget& ot, double k,
                                                 int w_begin, int w_end, int tpw, int wg_beam0) {
     constexpr int KS = A / 16;
-    constexpr int TILE_BYTES = KS * kWaveSize * 16;
-    constexpr int DW_ROW = A / 4;
-    constexpr int DW_TILE = 16 * DW_ROW;
-    constexpr int DW_STAGE = ST * DW_TILE;
+    using Map = StageMap<A, kThreads>;
+    constexpr int TILE_BYTES = Map::TILE_BYTES;
+    constexpr int STAGE_BYTES = ST * TILE_BYTES;
+    constexpr int DW_STAGE = ST * Map::DW_TILE;
     constexpr int DW_THREAD = (DW_STAGE + kThreads - 1) / kThreads;
-    __shared__ __attribute__((aligned(16))) unsigned char lds[ST * TILE_BYTES];
+    constexpr bool ALL_VALID = DW_STAGE % kThreads == 0;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[2 * STAGE_BYTES];
 
     const int S = da.S;
     const int B = da.B;
@@ -359,47 +374,87 @@ __device__ __forceinline__ void tracker_windows(const RowSource& rs, const float
     const bool has_beams = beam0 < B;
     const int q_end = (w_end - w_begin) * tpw;  // tiles of this call
 
-    unsigned pre[DW_THREAD];
-    auto load_stage = [&](int q0) {
-#pragma unroll
-        for (int kk = 0; kk < DW_THREAD; ++kk) {
-            const int q = tid + kk * kThreads;
-            unsigned v = 0u;
-            if (q < DW_STAGE) {
-                const int tl = q / DW_TILE;
-                const int rem = q - tl * DW_TILE;
-                const int rho = rem & 15;  // rows fastest: 16 lanes fill
-                const int j = rem >> 4;    // 16 consecutive LDS slots
-                const int qt = q0 + tl;
-                if (qt < q_end) {
-                    const int w = w_begin + qt / tpw;
-                    const int in_win = 16 * (qt % tpw) + rho;
-                    const long t = rs.t_base + (long)w * S + in_win;
-                    if (in_win < S && t < rs.t_limit)
-                        v = *reinterpret_cast<const unsigned*>(
-                            rs.x_f + (size_t)(t & rs.ring_mask) * rs.row_stride + 4 * j);
-                }
-            }
-            pre[kk] = v;
+    // Staging constants of this thread (StageMap).
+    const int tl0 = Map::tile0(wave);
+    const int rem0 = Map::rem0(tid);
+    const int rho = rem0 & 15;
+    const int lds_mine = Map::lds_offset(tl0, rem0);
+    const size_t row_stride = rs.row_stride;
+    const long mask = rs.ring_mask;
+    const uint8_t* const x_thread = rs.x_f + 4 * (rem0 >> 4);
+    const uint8_t* const x_row = x_thread + (size_t)rho * row_stride;  // + tile base
+
+    // Tile q = (wl, jt): window w_begin + wl of the call, tile jt of it.
+    auto tile_ref = [&](int q, int wl, int jt) -> TileRef {
+        TileRef r;
+        r.t = rs.t_base + (long)(w_begin + wl) * S + 16 * jt;
+        long rows = S - 16 * jt;
+        rows = rows < 16 ? rows : 16;
+        const long room = rs.t_limit - r.t;
+        rows = rows < room ? rows : room;
+        r.rows = q < q_end && rows > 0 ? (int)rows : 0;
+        return r;
+    };
+    auto next_tile = [&](int& wl, int& jt) {
+        if (++jt == tpw) {
+            jt = 0;
+            ++wl;
         }
     };
-    auto store_stage = [&]() {
+    // Stage loads: the cursor (lw, lj) names the stage's first tile q0;
+    // dword kk of this thread belongs to tile q0 + tl0 + tld(kk).
+    int lw = 0, lj = 0;
+    auto ref_at = [&](int q0, int i) -> TileRef {
+        int wl = lw, jt = lj;
+        for (int m = 0; m < i; ++m) next_tile(wl, jt);
+        return tile_ref(q0 + i, wl, jt);
+    };
+    auto dword_valid = [&](int kk) -> bool { return ALL_VALID || tl0 + Map::tld(kk) < ST; };
+    // Loads the stage at q0 into dst and moves the cursor on by ST tiles.
+    // Common case: the stage lies inside one window, all its tiles are
+    // full and it does not wrap the ring -- tile i then starts 16 i rows
+    // after the first, and no row needs a guard.
+    auto load_stage = [&](int q0, unsigned (&dst)[DW_THREAD]) {
+        const long t0 = rs.t_base + (long)(w_begin + lw) * S + 16 * lj;
+        const bool fast = lj + ST <= tpw && 16 * (lj + ST) <= S && q0 + ST <= q_end &&
+                          t0 + 16 * ST <= rs.t_limit &&
+                          (mask < 0 || (t0 & mask) <= mask + 1 - 16 * ST);
+        if (fast) {
+            const uint8_t* const x0 = x_row + (size_t)(t0 & mask) * row_stride;
+#pragma unroll
+            for (int kk = 0; kk < DW_THREAD; ++kk) {
+                const size_t off = (size_t)(16 * (tl0 + Map::tld(kk))) * row_stride +
+                                   Map::byte_delta(kk);
+                dst[kk] = dword_valid(kk) ? *reinterpret_cast<const unsigned*>(x0 + off) : 0u;
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < DW_THREAD; ++kk) {
+                const TileRef r = ref_at(q0, tl0 + Map::tld(kk));
+                unsigned v = 0u;
+                if (dword_valid(kk) && rho < r.rows) {
+                    const size_t row = (size_t)((r.t + rho) & mask);
+                    v = *reinterpret_cast<const unsigned*>(x_thread + row * row_stride +
+                                                           Map::byte_delta(kk));
+                }
+                dst[kk] = v;
+            }
+        }
+        // move the cursor on by ST tiles
+        lj += ST;
+        if (lj >= tpw) {
+            lw += lj / tpw;
+            lj %= tpw;
+        }
+    };
+    auto store_stage = [&](const unsigned (&src)[DW_THREAD], unsigned char* buf) {
 #pragma unroll
         for (int kk = 0; kk < DW_THREAD; ++kk) {
-            const int q = tid + kk * kThreads;
-            if (q < DW_STAGE) {
-                const int tl = q / DW_TILE;
-                const int rem = q - tl * DW_TILE;
-                const int rho = rem & 15;  // rows fastest: 16 lanes fill
-                const int j = rem >> 4;    // 16 consecutive LDS slots
-                const int g = j / (A / 16);
-                const int s = j - g * (A / 16);
-                unsigned h2[4];
-                bf_int4x4_to_half2x4(pre[kk], h2);
-                bf_uint4v u = {h2[0], h2[1], h2[2], h2[3]};
-                *reinterpret_cast<bf_uint4v*>(lds + tl * TILE_BYTES +
-                                              (s * kWaveSize + rho + 16 * g) * 16) = u;
-            }
+            if (!dword_valid(kk)) continue;
+            unsigned h2[4];
+            bf_int4x4_to_half2x4(src[kk], h2);
+            bf_uint4v u = {h2[0], h2[1], h2[2], h2[3]};
+            *reinterpret_cast<bf_uint4v*>(buf + lds_mine + Map::lds_delta(kk)) = u;
         }
     };
 
@@ -412,27 +467,34 @@ __device__ __forceinline__ void tracker_windows(const RowSource& rs, const float
     for (int nt = 0; nt < NT; ++nt) win[nt] = 0.0f;
     const int col = lane & 15;
     const int G = lane >> 4;
+    const bool direct = S >= kDirectWeightMinSpectra;
 
-    if (q_end > 0) load_stage(0);
+    unsigned pre_a[DW_THREAD];  // stage s + 1
+    unsigned pre_b[DW_THREAD];  // stage s + 2
+    if (q_end <= 0) return;
+    load_stage(0, pre_a);
+    store_stage(pre_a, lds);
+    load_stage(ST, pre_a);
+    __syncthreads();
+    int cw = 0, cj = 0;  // compute cursor: tile q0 + tl
+    int buf = 0;
     for (int q0 = 0; q0 < q_end; q0 += ST) {
-        __syncthreads();
-        store_stage();
-        __syncthreads();
-        if (q0 + ST < q_end) load_stage(q0 + ST);
-        if (!has_beams) continue;
+        const unsigned char* cur = lds + buf * STAGE_BYTES;
+        load_stage(q0 + 2 * ST, pre_b);
 #pragma unroll 1
-        for (int tl = tile_first; tl < ST && q0 + tl < q_end; tl += per_block) {
+        for (int tl = 0; tl < ST; ++tl) {
             const int qt = q0 + tl;
-            const int wl = qt / tpw;        // window within this call
-            const int jt = qt - wl * tpw;   // tile within the window
-            const int w = w_begin + wl;
+            const int w = w_begin + cw;
+            const int jt = cj;
+            next_tile(cw, cj);
+            if (!has_beams || qt >= q_end || tl % per_block != tile_first) continue;
             if (jt == 0)
-                window_operands<A, NT, SPLIT, DIRSRC, ANCHORED>(
-                    da, s_pos, k, w, beam0, lane, S >= kExactWeightMinSpectra, anc, bhi, blo);
+                window_operands<A, NT, SPLIT, DIRSRC, ANCHORED>(da, s_pos, k, w, beam0, lane,
+                                                                direct, anc, bhi, blo);
             bf_half8 av[KS];
 #pragma unroll
             for (int s = 0; s < KS; ++s)
-                av[s] = *reinterpret_cast<const bf_half8*>(lds + tl * TILE_BYTES +
+                av[s] = *reinterpret_cast<const bf_half8*>(cur + tl * TILE_BYTES +
                                                            (s * kWaveSize + lane) * 16);
             bf_float4 acc[NT];
 #pragma unroll
@@ -488,6 +550,13 @@ __device__ __forceinline__ void tracker_windows(const RowSource& rs, const float
                 }
             }
         }
+        // Stage s + 1 into the other buffer (last read before the previous
+        // barrier); stage s + 2's loads move up.
+        store_stage(pre_a, lds + (buf ^ 1) * STAGE_BYTES);
+#pragma unroll
+        for (int kk = 0; kk < DW_THREAD; ++kk) pre_a[kk] = pre_b[kk];
+        __syncthreads();
+        buf ^= 1;
     }
 }
 
@@ -787,8 +856,10 @@ __device__ __forceinline__ void serve_short_task(const uint8_t* __restrict__ rin
 // Items: persistent workgroups first share the workgroup items of
 // multi-tile-window slots (four waves cover up to 64 beams), then each wave
 // takes wave tasks of one-tile-window slots on its own.
+// Two workgroups per CU (the persistent grid) need at most 256 registers.
 template <int A, bool SPLIT>
-__global__ __launch_bounds__(kThreads) void serve_items_kernel(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+void serve_items_kernel(
     const uint8_t* __restrict__ ring, long long ring_mask, int F, Geom geo,
     const ServeSlot* __restrict__ slots, const ServeWork* __restrict__ work) {
     __shared__ int off_l[kServeMaxSlots + 1];
@@ -850,7 +921,7 @@ __global__ __launch_bounds__(kThreads) void serve_items_kernel(
 template <int A, int NT, bool SPLIT, bool INTEGRATE, int DIRSRC, bool ANCHORED>
 void launch_mfma_nt(hipStream_t stream, const TrackerLaunch& L, const DirArgs& da) {
     const int tpw = (int)ceil_div((size_t)L.S, 16);
-    constexpr int ST = 4;
+    constexpr int ST = A == 256 ? 2 : 4;  // two stage buffers of ST tiles in LDS
     // Chunk: ~1280 spectra of whole windows per workgroup.
     int wpc = (int)ceil_div(80, (size_t)tpw);
     if (wpc < 1) wpc = 1;
@@ -866,14 +937,14 @@ void launch_mfma_nt(hipStream_t stream, const TrackerLaunch& L, const DirArgs& d
 }
 
 // Beams per wave: as many 8-beam N-tiles as the beam count needs, up to 4
-// (2 at A=128, 1 at A=256) with FP64-sincos weights; short windows keep
+// (2 at A=128, 1 at A=256) with directly evaluated weights; short windows keep
 // anchor phasors in registers and take at most 2.
 template <int A, bool SPLIT, bool INTEGRATE, int DIRSRC>
 void launch_mfma_a(hipStream_t stream, const TrackerLaunch& L, const DirArgs& da) {
     constexpr int kMaxNT = A <= 64 ? 4 : (A == 128 ? 2 : 1);
     constexpr int kMaxNT2 = kMaxNT >= 2 ? 2 : 1;
     const size_t need = ceil_div(L.B, (size_t)(kWaves * 8));
-    if (L.S < (size_t)kExactWeightMinSpectra) {
+    if (L.S < (size_t)kDirectWeightMinSpectra) {
         if (need <= 1 || kMaxNT2 == 1)
             launch_mfma_nt<A, 1, SPLIT, INTEGRATE, DIRSRC, true>(stream, L, da);
         else

@@ -42,6 +42,7 @@
 #include <type_traits>
 
 #include "bf_kernels.hpp"
+#include "mfma_stage.hpp"
 
 namespace bfhip {
 
@@ -110,28 +111,6 @@ __device__ __forceinline__ void load_weights(const float2* __restrict__ w, int F
     }
 }
 
-// Staging map of one stage: dword k of a thread is q = tid + k*kThreads of
-// the stage, i.e. tile tl0 + tld(k) and in-tile dword rem0 + remd(k), where
-// tl0, rem0 depend on the thread only and tld, remd are compile-time (both
-// kThreads and the tile's dword count 4A are powers of two).
-template <int A, int kThreads>
-struct StageMap {
-    static constexpr int DW_TILE = 4 * A;
-    static constexpr int TILE_BYTES = (A / 16) * kWaveSize * 16;
-    static constexpr int tld(int k) {
-        return kThreads >= DW_TILE ? k * (kThreads / DW_TILE) : (k * kThreads) / DW_TILE;
-    }
-    static constexpr int remd(int k) {
-        return kThreads >= DW_TILE ? 0 : (k * kThreads) % DW_TILE;
-    }
-    // dword j of a row sits in k-step j % (A/16) of lane group j / (A/16);
-    // remd is a multiple of 256, so it moves the lane group only.
-    static constexpr int lds_delta(int k) {
-        return tld(k) * TILE_BYTES + 16 * 16 * ((remd(k) >> 4) / (A / 16));
-    }
-    static constexpr int byte_delta(int k) { return 4 * (remd(k) >> 4); }
-};
-
 // At least two waves per SIMD (at most 256 registers per lane): the
 // unrolled stage otherwise keeps several tiles' accumulators in flight.
 template <int A, int NT, int WV, bool SPLIT, int MODE, int ST, bool DB>
@@ -182,21 +161,15 @@ void mfma_fused_kernel(
     const int tps = MODE == 2 ? (S >> 2) : 5;           // tiles per super-tile
     const int pitch = MODE == 0 ? 4 : (MODE == 1 ? 20 : S);  // spectra between row groups
 
-    // Per-thread staging constants (rows fastest within 16 lanes, so the
-    // 16-byte LDS slots s*64 + row + 16g written by 16 consecutive lanes are
-    // contiguous and each global load reads 16 contiguous bytes per row).
-    // tl0 is uniform per wave (a tile's dwords span whole waves)
-    const int tl0 = kThreads >= DW_TILE
-                        ? __builtin_amdgcn_readfirstlane(wave * kWaveSize / DW_TILE)
-                        : 0;
-    const int rem0 = kThreads >= DW_TILE ? tid % DW_TILE : tid;
+    // Per-thread staging constants (StageMap: rows fastest within 16 lanes).
+    const int tl0 = Map::tile0(wave);  // uniform per wave
+    const int rem0 = Map::rem0(tid);
     const int rho = rem0 & 15;
     const int j0 = rem0 >> 4;
     // spectrum of this thread's row within its tile's row group pattern
     const int prow = (rho >> 2) * pitch + (rho & 3);
     const uint8_t* xs = x + (size_t)f * A + (size_t)prow * row_stride + 4 * j0;
-    unsigned char* const lds_mine =
-        lds + tl0 * TILE_BYTES + ((j0 % KS) * kWaveSize + rho + 16 * (j0 / KS)) * 16;
+    const int lds_mine = Map::lds_offset(tl0, rem0);
 
     // First spectrum of the stage starting at tile st0, and offset of tile
     // tl of a stage from it (row group 0, row 0).
@@ -236,7 +209,7 @@ void mfma_fused_kernel(
         unsigned h2[4];
         bf_int4x4_to_half2x4(pre[k], h2);
         bf_uint4v u = {h2[0], h2[1], h2[2], h2[3]};
-        *reinterpret_cast<bf_uint4v*>(buf + (lds_mine - lds) + Map::lds_delta(k)) = u;
+        *reinterpret_cast<bf_uint4v*>(buf + lds_mine + Map::lds_delta(k)) = u;
     };
     auto load_stage = [&](int st0) {
         const int ts = stage_time(st0);
