@@ -481,75 +481,95 @@ __device__ __forceinline__ void tracker_windows(const RowSource& rs, const float
     for (int q0 = 0; q0 < q_end; q0 += ST) {
         const unsigned char* cur = lds + buf * STAGE_BYTES;
         load_stage(q0 + 2 * ST, pre_b);
-#pragma unroll 1
-        for (int tl = 0; tl < ST; ++tl) {
-            const int qt = q0 + tl;
+        // The stage in segments of consecutive tiles of one window: the
+        // weights at a window's first tile, a check-free tile loop, the
+        // window's sum at its last tile.  When waves share a block
+        // (one-tile windows), a segment is one tile, taken by one wave.
+        int tl = 0;
+        while (tl < ST && q0 + tl < q_end) {
             const int w = w_begin + cw;
-            const int jt = cj;
-            next_tile(cw, cj);
-            if (!has_beams || qt >= q_end || tl % per_block != tile_first) continue;
-            if (jt == 0)
-                window_operands<A, NT, SPLIT, DIRSRC, ANCHORED>(da, s_pos, k, w, beam0, lane,
-                                                                direct, anc, bhi, blo);
-            bf_half8 av[KS];
+            const int jt0 = cj;
+            int seg = tpw - jt0;
+            seg = seg < ST - tl ? seg : ST - tl;
+            seg = seg < q_end - q0 - tl ? seg : q_end - q0 - tl;
+            if (has_beams && tl % per_block == tile_first) {
+                if (jt0 == 0)
+                    window_operands<A, NT, SPLIT, DIRSRC, ANCHORED>(da, s_pos, k, w, beam0, lane,
+                                                                    direct, anc, bhi, blo);
+#pragma unroll 1
+                for (int i = 0; i < seg; ++i) {
+                    const unsigned char* tile = cur + (tl + i) * TILE_BYTES;
+                    bf_half8 av[KS];
 #pragma unroll
-            for (int s = 0; s < KS; ++s)
-                av[s] = *reinterpret_cast<const bf_half8*>(cur + tl * TILE_BYTES +
-                                                           (s * kWaveSize + lane) * 16);
-            bf_float4 acc[NT];
+                    for (int s = 0; s < KS; ++s)
+                        av[s] = *reinterpret_cast<const bf_half8*>(tile +
+                                                                   (s * kWaveSize + lane) * 16);
+                    bf_float4 acc[NT];
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[nt] = bf_float4{0.0f, 0.0f, 0.0f, 0.0f};
+                    for (int nt = 0; nt < NT; ++nt) acc[nt] = bf_float4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int s = 0; s < KS; ++s) {
+                    for (int s = 0; s < KS; ++s) {
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[s], bhi[nt][s], acc[nt],
-                                                                     0, 0, 0);
-                if (SPLIT) {
+                        for (int nt = 0; nt < NT; ++nt)
+                            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                                av[s], bhi[nt][s], acc[nt], 0, 0, 0);
+                        if (SPLIT) {
 #pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
-                        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[s], blo[nt][s],
-                                                                         acc[nt], 0, 0, 0);
-                }
-            }
-            if (!INTEGRATE) {
+                            for (int nt = 0; nt < NT; ++nt)
+                                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                                    av[s], blo[nt][s], acc[nt], 0, 0, 0);
+                        }
+                    }
+                    if (!INTEGRATE) {
+                        const int jt = jt0 + i;
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    const int b = beam0 + nt * 8 + (col >> 1);
+                        for (int nt = 0; nt < NT; ++nt) {
+                            const int b = beam0 + nt * 8 + (col >> 1);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const float sq = acc[nt][r] * acc[nt][r];
-                        const float p = sq + dpp_swap_pairs(sq);
-                        const int in_win = 16 * jt + 4 * G + r;
-                        const long t = (long)w * S + in_win;
-                        if ((col & 1) == 0 && b < B && in_win < S && rs.t_base + t < rs.t_limit)
-                            ot.out[((size_t)t * F + f) * B + b] = p;
+                            for (int r = 0; r < 4; ++r) {
+                                const float sq = acc[nt][r] * acc[nt][r];
+                                const float p = sq + dpp_swap_pairs(sq);
+                                const int in_win = 16 * jt + 4 * G + r;
+                                const long t = (long)w * S + in_win;
+                                if ((col & 1) == 0 && b < B && in_win < S &&
+                                    rs.t_base + t < rs.t_limit)
+                                    ot.out[((size_t)t * F + f) * B + b] = p;
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int nt = 0; nt < NT; ++nt) {
+                            float a = win[nt];
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) a = fmaf(acc[nt][r], acc[nt][r], a);
+                            win[nt] = a;
+                        }
                     }
                 }
-                continue;
-            }
+                if (INTEGRATE && jt0 + seg == tpw) {
+                    const long slot = ot.out_windows > 0 ? (long)w % ot.out_windows : (long)w;
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                float a = win[nt];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) a = fmaf(acc[nt][r], acc[nt][r], a);
-                win[nt] = a;
-            }
-            if (jt == tpw - 1) {
-                const long slot = ot.out_windows > 0 ? (long)w % ot.out_windows : (long)w;
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    float v = win[nt];
-                    v += __shfl_xor(v, 16);
-                    v += __shfl_xor(v, 32);
-                    v += dpp_swap_pairs(v);
-                    const int b = beam0 + nt * 8 + (col >> 1);
-                    if (G == 0 && (col & 1) == 0 && b < B)
-                        ot.out[((size_t)slot * F + f) * B + b] = v;
-                    win[nt] = 0.0f;
+                    for (int nt = 0; nt < NT; ++nt) {
+                        float v = win[nt];
+                        v += __shfl_xor(v, 16);
+                        v += __shfl_xor(v, 32);
+                        v += dpp_swap_pairs(v);
+                        const int b = beam0 + nt * 8 + (col >> 1);
+                        if (G == 0 && (col & 1) == 0 && b < B)
+                            ot.out[((size_t)slot * F + f) * B + b] = v;
+                        win[nt] = 0.0f;
+                    }
                 }
             }
+            tl += seg;
+            cj += seg;
+            if (cj == tpw) {
+                cj = 0;
+                ++cw;
+            }
         }
+        // the cursor stays on the stage grid when the call ends mid-stage
+        for (; tl < ST; ++tl) next_tile(cw, cj);
         // Stage s + 1 into the other buffer (last read before the previous
         // barrier); stage s + 2's loads move up.
         store_stage(pre_a, lds + (buf ^ 1) * STAGE_BYTES);
