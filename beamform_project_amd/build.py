@@ -3,7 +3,8 @@
     python -m beamform_project_amd.build
 
 Objects are compiled in parallel into beamform_project_amd/_build/ and
-linked into beamform_project_amd/libbeamform_hip.so.  Every call compiles
+linked into beamform_project_amd/libbeamform_hip.so; the command-line tools
+in cli/ are then built into beamform_project_amd/bin/.  Every call compiles
 (the driver's build check relies on that); object files are reused only
 when their source and the shared headers are older than the object.
 """
@@ -85,7 +86,32 @@ def build(force: bool = False, jobs: int | None = None) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("link of libbeamform_hip.so failed")
     shutil.move(tmp, LIB)
+    build_cli()
     return LIB
+
+
+CLI = os.path.join(PKG, "cli")
+BIN = os.path.join(PKG, "bin")
+CXX = os.environ.get("CXX", "g++")
+
+
+def build_cli() -> list:
+    """Compiles the command-line tools in cli/ against the library (host
+    compiler; they use the C++ layer in include/beamformer/ only)."""
+    os.makedirs(BIN, exist_ok=True)
+    out = []
+    for src in sorted(f for f in os.listdir(CLI) if f.endswith(".cpp")):
+        exe = os.path.join(BIN, src[:-4])
+        cmd = [CXX, "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wextra",
+               "-I", os.path.join(ROOT, "include"), os.path.join(CLI, src), "-o", exe + ".tmp",
+               "-L", PKG, "-lbeamform_hip", "-Wl,-rpath,$ORIGIN/.."]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"build of {src} failed")
+        shutil.move(exe + ".tmp", exe)
+        out.append(exe)
+    return out
 
 
 if __name__ == "__main__":

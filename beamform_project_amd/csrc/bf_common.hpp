@@ -81,6 +81,20 @@ inline void validate_dimensions(const bf_dims& d) {
         invalid("n_beams must be between 1 and 128");
 }
 
+inline constexpr size_t kFrequencyShardCount = 2;  // config.hpp:13
+
+// Reference include/beamformer/config.hpp:43-63 (validate_shard_descriptor).
+inline void validate_shard(const bf_shard& s) {
+    if (s.shard_count != kFrequencyShardCount) invalid("shard_count must be exactly 2");
+    if (s.shard_id >= s.shard_count) invalid("shard_id is outside the two-shard band");
+    if (s.local_frequency_count != kLocalFrequencyChannels)
+        invalid("local_frequency_count must be exactly 336");
+    if (s.absolute_frequency_start > (kFrequencyShardCount - 1) * kLocalFrequencyChannels)
+        invalid("absolute frequency start is outside the full band");
+    if (s.timestamp_step == 0) invalid("timestamp_step must be positive");
+    if (!s.loss_mask_independent) invalid("loss masks must be independent per shard");
+}
+
 inline const bf_dims& deref_dims(const bf_dims* d) {
     if (d == nullptr) invalid("dims must not be null");
     return *d;

@@ -179,17 +179,6 @@ size_t dtype_size(int t) {
     }
 }
 
-// Reference include/beamformer/config.hpp:43-63.
-void validate_shard(const bf_shard& s) {
-    if (s.shard_count != 2) invalid("shard_count must be exactly 2");
-    if (s.shard_id >= s.shard_count) invalid("shard_id is outside the two-shard band");
-    if (s.local_frequency_count != 336) invalid("local_frequency_count must be exactly 336");
-    if (s.absolute_frequency_start > 672 - 336)
-        invalid("absolute frequency start is outside the full band");
-    if (s.timestamp_step == 0) invalid("timestamp_step must be positive");
-    if (!s.loss_mask_independent) invalid("loss masks must be independent per shard");
-}
-
 bool same_shard(const bf_shard& a, const bf_shard& b) {
     return a.shard_id == b.shard_id && a.shard_count == b.shard_count &&
            a.local_frequency_count == b.local_frequency_count &&

@@ -136,6 +136,30 @@ inline std::vector<float> beamform_packed_integrated_intensity(
     return out;
 }
 
+// cuda_beamform_packed_quantized_integrated_intensity (cuda_beamformer.hpp:
+// 113-123): int8 codes [w][f][b] and {offset, scale} pairs per 1x16x16 chunk.
+struct QuantizedIntegratedIntensity {
+    std::vector<std::int8_t> codes;
+    std::vector<float> parameters;  // {offset, scale} pairs
+};
+
+inline QuantizedIntegratedIntensity beamform_packed_quantized_integrated_intensity(
+    const std::vector<std::uint8_t>& packed, const std::vector<float>& weights,
+    const Dimensions& d, TemporalIntegrationConfig cfg, Kernel kernel = Kernel::Tiled,
+    bf_timings* timings = nullptr) {
+    const bf_dims c = d.c();
+    const Dimensions out_dims{integrated_time_count(d.n_time, cfg.integration_spectra),
+                              d.n_freq, d.n_ant, d.n_beams};
+    QuantizedIntegratedIntensity out;
+    out.codes.resize(out_dims.n_time * d.n_freq * d.n_beams);
+    out.parameters.resize(2 * quantization_parameter_count(out_dims));
+    check(bf_beamform_packed_quantized_integrated_intensity(
+        packed.data(), packed.size(), weights.data(), weights.size() / 2, &c,
+        cfg.integration_spectra, static_cast<int>(kernel), out.codes.data(), out.codes.size(),
+        out.parameters.data(), out.parameters.size() / 2, timings));
+    return out;
+}
+
 // Reference TrackerConfig: direction at sample 0 and linear drift per sample.
 struct TrackerConfig {
     float direction_start[3] = {0.0f, 0.0f, 1.0f};

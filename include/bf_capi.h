@@ -620,6 +620,77 @@ bf_status bf_generate_point_source_noise(void* stream, uint8_t* out, size_t n_ti
                                          const float* direction3, float amplitude,
                                          float sigma, uint64_t seed);
 
+/* --------------------------------------------------- on-disk formats
+ * Host-only readers and writers of the reference's files, byte for byte:
+ * raw little-endian arrays whose size must match the dimensions exactly,
+ * and key=value metadata text.  Reference: include/beamformer/io.hpp,
+ * src/io.cpp:60-342, include/beamformer/formats.hpp:13-77.  IO failures
+ * (cannot open, wrong file size, missing or unsupported metadata field)
+ * return BF_ERR_RUNTIME as the reference throws std::runtime_error; size
+ * and contract violations return BF_ERR_INVALID_ARGUMENT. */
+
+/* Reference include/beamformer/quantization.hpp:75-81
+ * (QuantizedOutputMetadata); parameters_file is a NUL-terminated name. */
+typedef struct bf_quantized_output_metadata {
+    bf_dims output_dims;
+    size_t input_n_time;
+    size_t integration_spectra;
+    bf_shard shard;
+    char parameters_file[1024];
+} bf_quantized_output_metadata;
+
+/* default_shard_descriptors()[shard_id], config.hpp:65-72. */
+bf_status bf_default_shard_descriptor(size_t shard_id, bf_shard* out);
+/* validate_shard_descriptor, config.hpp:43-63. */
+bf_status bf_validate_shard_descriptor(const bf_shard* shard);
+
+/* read_packed_voltage / write_packed_voltage, io.cpp:60-72: [t][f][a] bytes. */
+bf_status bf_read_packed_voltage(const char* path, const bf_dims* dims, uint8_t* out,
+                                 size_t out_bytes);
+bf_status bf_write_packed_voltage(const char* path, const uint8_t* voltage, size_t bytes,
+                                  const bf_dims* dims);
+/* read_weights / write_weights, io.cpp:74-86: [b][f][a] complex f32. */
+bf_status bf_read_weights(const char* path, const bf_dims* dims, float* out,
+                          size_t out_complex);
+bf_status bf_write_weights(const char* path, const float* weights, size_t n_complex,
+                           const bf_dims* dims);
+/* read_tiled_weights / write_tiled_weights, io.cpp:88-101. */
+bf_status bf_read_tiled_weights(const char* path, const bf_dims* dims, float* out,
+                                size_t out_complex);
+bf_status bf_write_tiled_weights(const char* path, const float* weights, size_t n_complex,
+                                 const bf_dims* dims);
+/* write_intensities, io.cpp:103-111: [t][f][b] f32. */
+bf_status bf_write_intensities(const char* path, const float* intensity, size_t count,
+                               const bf_dims* dims);
+/* read/write_quantized_intensities, io.cpp:113-129: [w][f][b] int8. */
+bf_status bf_read_quantized_intensities(const char* path, const bf_dims* dims, int8_t* out,
+                                        size_t out_count);
+bf_status bf_write_quantized_intensities(const char* path, const int8_t* codes, size_t count,
+                                         const bf_dims* dims);
+/* read/write_quantization_parameters, io.cpp:131-146: {offset, scale} f32. */
+bf_status bf_read_quantization_parameters(const char* path, const bf_dims* integrated_dims,
+                                          float* out, size_t out_params);
+bf_status bf_write_quantization_parameters(const char* path, const float* params,
+                                           size_t n_params, const bf_dims* integrated_dims);
+/* beamformer-quantized-int8-v1 metadata, io.cpp:149-242. */
+bf_status bf_write_quantized_output_metadata(const char* path,
+                                             const bf_quantized_output_metadata* metadata);
+bf_status bf_read_quantized_output_metadata(const char* path,
+                                            bf_quantized_output_metadata* out);
+/* write_packed_shard / read_packed_shard, io.cpp:244-342: payload bytes,
+ * one validity byte (0 or 1) per [t][f] frame, beamformer-packed-shard-v1
+ * metadata. */
+bf_status bf_write_packed_shard(const char* payload_path, const char* metadata_path,
+                                const char* loss_mask_path, const bf_shard* shard,
+                                const uint8_t* payload, size_t payload_bytes,
+                                const uint8_t* loss_mask, size_t loss_mask_bytes,
+                                const bf_dims* dims);
+bf_status bf_read_packed_shard(const char* payload_path, const char* metadata_path,
+                               const char* loss_mask_path, const bf_dims* dims,
+                               bf_shard* shard_out, uint8_t* payload,
+                               size_t payload_capacity, uint8_t* loss_mask,
+                               size_t loss_mask_capacity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -339,6 +339,14 @@ def ref_lib():
         "ref_beam_tracker_opt_v2": [_u8p, _sz, _sz, _sz, C.c_float, C.c_float, C.c_float,
                                     C.c_float, _sz, _f32p],
         "ref_tracker_direction": [C.c_float, C.c_float, C.c_float, C.c_float, _sz, _f32p],
+        "ref_write_quantized_metadata": [C.c_char_p, _sz, _sz, _sz, _sz, _sz, _sz, _sz,
+                                         C.c_char_p],
+        "ref_read_quantized_metadata": [C.c_char_p, C.POINTER(C.c_uint64)],
+        "ref_write_packed_shard": [C.c_char_p, C.c_char_p, C.c_char_p, _u8p, _u8p, _sz, _sz,
+                                   _sz, _sz, _sz, C.c_uint64, C.c_uint64],
+        "ref_read_packed_shard": [C.c_char_p, C.c_char_p, C.c_char_p, _sz, _sz, _sz, _sz,
+                                  _u8p, _u8p, C.POINTER(C.c_uint64)],
+        "ref_write_tiled_weights_file": [C.c_char_p, _f32p, _sz, _sz, _sz, _sz],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)

@@ -8,6 +8,7 @@
 #include "beamformer/beam_tracker_opt_v2.hpp"
 #include "beamformer/cpu_beamformer.hpp"
 #include "beamformer/geometry.hpp"
+#include "beamformer/io.hpp"
 #include "beamformer/quantization.hpp"
 #include "beamformer/synthetic_data.hpp"
 #include "beamformer/temporal_integration.hpp"
@@ -204,6 +205,82 @@ int ref_tracker_direction(float l0, float m0, float rl, float rm, std::size_t t,
     return guarded([&] {
         TrackerTrajectoryConfig tr{direction_from_lm(l0, m0), {rl, rm}};
         put_vec3s({tracker_direction(tr, t)}, out);
+    });
+}
+
+// ---- on-disk formats (src/io.cpp): the reference writes and reads files
+// so tests can compare them byte for byte with the library's.
+int ref_write_quantized_metadata(const char* path, std::size_t T, std::size_t F, std::size_t A,
+                                 std::size_t B, std::size_t input_n_time, std::size_t S,
+                                 std::size_t shard_id, const char* parameters_file) {
+    return guarded([&] {
+        const QuantizedOutputMetadata m{Dimensions{T, F, A, B}, input_n_time,
+                                        TemporalIntegrationConfig{S},
+                                        default_shard_descriptors()[shard_id],
+                                        parameters_file};
+        write_quantized_output_metadata(path, m);
+    });
+}
+
+// Reads the metadata at `path`; out = {n_time, n_freq, n_ant, n_beams,
+// input_n_time, integration_spectra, shard_id, absolute_frequency_start,
+// loss_mask_id, timestamp_start, timestamp_step}.
+int ref_read_quantized_metadata(const char* path, std::uint64_t* out) {
+    return guarded([&] {
+        const auto m = read_quantized_output_metadata(path);
+        const std::uint64_t v[11] = {m.output_dims.n_time, m.output_dims.n_freq,
+                                     m.output_dims.n_ant, m.output_dims.n_beams,
+                                     m.input_n_time, m.temporal_integration.integration_spectra,
+                                     m.shard.shard_id, m.shard.absolute_frequency_start,
+                                     m.shard.loss_mask_id, m.shard.timestamp_start,
+                                     m.shard.timestamp_step};
+        std::memcpy(out, v, sizeof(v));
+    });
+}
+
+int ref_write_packed_shard(const char* payload_path, const char* metadata_path,
+                           const char* mask_path, const std::uint8_t* payload,
+                           const std::uint8_t* mask, std::size_t T, std::size_t F,
+                           std::size_t A, std::size_t B, std::size_t shard_id,
+                           std::uint64_t timestamp_start, std::uint64_t timestamp_step) {
+    return guarded([&] {
+        const Dimensions d{T, F, A, B};
+        PackedShard sh;
+        sh.descriptor = default_shard_descriptors()[shard_id];
+        sh.descriptor.timestamp_start = timestamp_start;
+        sh.descriptor.timestamp_step = timestamp_step;
+        sh.payload.assign(payload, payload + T * F * A);
+        sh.loss_mask.assign(mask, mask + T * F);
+        write_packed_shard(payload_path, metadata_path, mask_path, sh, d);
+    });
+}
+
+int ref_read_packed_shard(const char* payload_path, const char* metadata_path,
+                          const char* mask_path, std::size_t T, std::size_t F, std::size_t A,
+                          std::size_t B, std::uint8_t* payload, std::uint8_t* mask,
+                          std::uint64_t* descriptor8) {
+    return guarded([&] {
+        const auto sh = read_packed_shard(payload_path, metadata_path, mask_path,
+                                          Dimensions{T, F, A, B});
+        std::memcpy(payload, sh.payload.data(), sh.payload.size());
+        std::memcpy(mask, sh.loss_mask.data(), sh.loss_mask.size());
+        const auto& s = sh.descriptor;
+        const std::uint64_t v[8] = {s.shard_id, s.shard_count, s.local_frequency_count,
+                                    s.absolute_frequency_start, s.timestamp_start,
+                                    s.timestamp_step, s.loss_mask_id,
+                                    s.loss_mask_independent ? 1u : 0u};
+        std::memcpy(descriptor8, v, sizeof(v));
+    });
+}
+
+int ref_write_tiled_weights_file(const char* path, const float* w, std::size_t T, std::size_t F,
+                                 std::size_t A, std::size_t B) {
+    return guarded([&] {
+        const Dimensions d{T, F, A, B};
+        const std::size_t n = tiled_weight_count(d);
+        TiledWeights tw(n);
+        std::memcpy(tw.data(), w, n * sizeof(ComplexFloat));
+        write_tiled_weights(path, tw, d);
     });
 }
 

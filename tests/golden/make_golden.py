@@ -13,6 +13,8 @@ Each case mirrors a case of the reference's own tests:
   integrate_ramp        tests/cpu/test_temporal_integration.cpp:59-90
   quantize_ramp         tests/cpu/test_quantization.cpp:38-62
   tracker_moving_a32    tests/cuda/test_cuda_tracker_v2.cpp:45-58 (shortened)
+  io_quantized_meta.txt src/io.cpp:149-183 (write_quantized_output_metadata)
+  io_shard_*.bin/.txt   src/io.cpp:244-278 (write_packed_shard, T=2 A=32)
 
 Run:  python tests/golden/make_golden.py
 """
@@ -122,8 +124,20 @@ def main() -> None:
     save("tracker_moving_a32", x=xm.reshape(T, F, A),
          trajectory=np.array([l0, m0, rl, rm], np.float32), spectra=np.int64(S),
          intensity=out)
+    # On-disk formats written by the reference's own io.cpp.
+    meta = os.path.join(HERE, "io_quantized_meta.txt")
+    # C2-like output: T=32768 -> 3277 windows of 10, A=64, B=128, shard 1
+    assert R.ref_write_quantized_metadata(meta.encode(), 3277, F, 64, 128, 32768, 10, 1,
+                                          b"c2_params.bin") == 0
+    T, A = 2, 32
+    payload = np.arange(T * F * A, dtype=np.uint32).astype(np.uint8)
+    mask = (np.arange(T * F) % 7 != 3).astype(np.uint8)
+    assert R.ref_write_packed_shard(os.path.join(HERE, "io_shard_payload.bin").encode(),
+                                    os.path.join(HERE, "io_shard_meta.txt").encode(),
+                                    os.path.join(HERE, "io_shard_mask.bin").encode(),
+                                    payload, mask, T, F, A, 5, 0, 1234567890123, 2560) == 0
     for name in sorted(os.listdir(HERE)):
-        if name.endswith(".npz"):
+        if name.endswith(".npz") or name.startswith("io_"):
             print(f"{name:32s} {os.path.getsize(os.path.join(HERE, name)):>9d} bytes")
 
 
