@@ -223,6 +223,26 @@ def launch_packed_integrated_beamformer(kernel: Kernel, stream, packed_voltage, 
          integration_spectra)
 
 
+def launch_packed_beamformer_masked(kernel: Kernel, stream, packed_voltage, loss_mask, weights,
+                                    intensity, dims: Dimensions) -> None:
+    """launch_packed_beamformer with a [T][F] device loss mask (0 = lost
+    frame, contributes nothing); MFMA kernels only (bf_launch_packed_beamformer_masked)."""
+    d = dims.c()
+    call("bf_launch_packed_beamformer_masked", int(kernel), _sh(stream), _ptr(packed_voltage),
+         _ptr(loss_mask), _ptr(weights), _ptr(intensity), C.byref(d))
+
+
+def launch_packed_integrated_beamformer_masked(kernel: Kernel, stream, packed_voltage,
+                                               loss_mask, weights, integrated_intensity,
+                                               dims: Dimensions,
+                                               integration_spectra: int) -> None:
+    """launch_packed_integrated_beamformer with a [T][F] device loss mask."""
+    d = dims.c()
+    call("bf_launch_packed_integrated_beamformer_masked", int(kernel), _sh(stream),
+         _ptr(packed_voltage), _ptr(loss_mask), _ptr(weights), _ptr(integrated_intensity),
+         C.byref(d), integration_spectra)
+
+
 def launch_quantize_integrated_intensity(stream, integrated_intensity, quantized_intensity,
                                          parameters, integrated_dims: Dimensions) -> None:
     """Reference include/beamformer/cuda_stage_api.hpp:36-42."""

@@ -21,8 +21,11 @@ void launch_fp32_fused(hipStream_t stream, const uint8_t* x, const float* w,
 // The tile shape comes from the autotuner's cache for this exact problem,
 // else from a default.
 bool mfma_supported(size_t A, size_t B, size_t S);
+// `mask`: optional [T][F] frame-validity bytes (0 = lost frame, whose
+// spectrum then contributes nothing); null when every frame is valid.
 void launch_mfma_fused(hipStream_t stream, bool split, const uint8_t* x, const float* w,
-                       float* out, size_t T, size_t F, size_t A, size_t B, size_t S);
+                       float* out, size_t T, size_t F, size_t A, size_t B, size_t S,
+                       const uint8_t* mask = nullptr);
 
 // Tile shape of one launch of the MFMA beamformer.
 struct MfmaConfig {

@@ -135,15 +135,21 @@ int resolve(int kernel, const bf_dims& d, size_t S) {
     return kernel;
 }
 
+// `mask` (optional): [T][F] frame-validity bytes; only the MFMA kernels
+// apply it.
 void run_beamformer(int kernel, hipStream_t stream, const uint8_t* x, const float* w,
-                    float* out, const bf_dims& d, size_t S) {
+                    float* out, const bf_dims& d, size_t S, const uint8_t* mask = nullptr) {
     require_ptr(x, "packed_voltage");
     require_ptr(w, "weights");
     require_ptr(out, "output");
     require_aligned(x, 16, "packed_voltage");
     require_aligned(w, 8, "weights");
     require_aligned(out, 4, "output");
-    switch (resolve(kernel, d, S)) {
+    const int resolved = resolve(kernel, d, S);
+    if (mask != nullptr && resolved != BF_KERNEL_SPLIT_F16 && resolved != BF_KERNEL_F16)
+        invalid("loss masks are applied by the MFMA kernels (SPLIT_F16, F16, or AUTO where "
+                "it selects one)");
+    switch (resolved) {
     case BF_KERNEL_DIRECT:
     case BF_KERNEL_FP32:
         launch_fp32_fused(stream, x, w, WeightLayout::Canonical, out, d.n_time, d.n_freq,
@@ -154,10 +160,12 @@ void run_beamformer(int kernel, hipStream_t stream, const uint8_t* x, const floa
                           d.n_beams, S);
         return;
     case BF_KERNEL_SPLIT_F16:
-        launch_mfma_fused(stream, true, x, w, out, d.n_time, d.n_freq, d.n_ant, d.n_beams, S);
+        launch_mfma_fused(stream, true, x, w, out, d.n_time, d.n_freq, d.n_ant, d.n_beams, S,
+                          mask);
         return;
     case BF_KERNEL_F16:
-        launch_mfma_fused(stream, false, x, w, out, d.n_time, d.n_freq, d.n_ant, d.n_beams, S);
+        launch_mfma_fused(stream, false, x, w, out, d.n_time, d.n_freq, d.n_ant, d.n_beams, S,
+                          mask);
         return;
     default:
         invalid("unknown beamformer kernel selector");
@@ -305,6 +313,35 @@ bf_status bf_launch_packed_integrated_beamformer(int kernel, void* stream,
         validate_integration(kernel, S);
         run_beamformer(kernel, static_cast<hipStream_t>(stream), packed, weights, integrated, d,
                        S);
+    });
+}
+
+bf_status bf_launch_packed_beamformer_masked(int kernel, void* stream, const uint8_t* packed,
+                                             const uint8_t* loss_mask, const float* weights,
+                                             float* intensity, const bf_dims* dims) {
+    return guarded([&] {
+        validate_kernel(kernel);
+        const bf_dims& d = deref_dims(dims);
+        validate_dimensions(d);
+        require_ptr(loss_mask, "loss_mask");
+        run_beamformer(kernel, static_cast<hipStream_t>(stream), packed, weights, intensity, d,
+                       0, loss_mask);
+    });
+}
+
+bf_status bf_launch_packed_integrated_beamformer_masked(int kernel, void* stream,
+                                                        const uint8_t* packed,
+                                                        const uint8_t* loss_mask,
+                                                        const float* weights, float* integrated,
+                                                        const bf_dims* dims, size_t S) {
+    return guarded([&] {
+        validate_kernel(kernel);
+        const bf_dims& d = deref_dims(dims);
+        validate_dimensions(d);
+        validate_integration(kernel, S);
+        require_ptr(loss_mask, "loss_mask");
+        run_beamformer(kernel, static_cast<hipStream_t>(stream), packed, weights, integrated, d,
+                       S, loss_mask);
     });
 }
 

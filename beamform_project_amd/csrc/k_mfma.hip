@@ -103,7 +103,8 @@ bool mfma_supported(size_t A, size_t B, size_t S) {
 }
 
 void launch_mfma_fused(hipStream_t stream, bool split, const uint8_t* x, const float* w,
-                       float* out, size_t T, size_t F, size_t A, size_t B, size_t S) {
+                       float* out, size_t T, size_t F, size_t A, size_t B, size_t S,
+                       const uint8_t* mask) {
     check_problem(T, F, A, B, S);
     MfmaConfig cfg = default_config((int)A, (int)B, split);
     {
@@ -111,7 +112,7 @@ void launch_mfma_fused(hipStream_t stream, bool split, const uint8_t* x, const f
         auto it = tune_cache().find(TuneKey((int)T, (int)F, (int)A, (int)B, (int)S, split));
         if (it != tune_cache().end()) cfg = it->second;
     }
-    const MfmaLaunch L{x, w, out, (int)T, (int)F, (int)B, (int)S, split};
+    const MfmaLaunch L{x, w, out, mask, (int)T, (int)F, (int)B, (int)S, split};
     if (!launch_cfg(stream, L, (int)A, cfg)) invalid("MFMA tile shape is not instantiated");
     hip_check(hipGetLastError(), "mfma_fused_kernel launch");
 }
@@ -122,7 +123,7 @@ std::vector<MfmaTuneResult> autotune_mfma_fused(hipStream_t stream, bool split,
                                                 size_t S, int repeats) {
     check_problem(T, F, A, B, S);
     if (repeats < 1) invalid("repeats must be positive");
-    const MfmaLaunch L{x, w, out, (int)T, (int)F, (int)B, (int)S, split};
+    const MfmaLaunch L{x, w, out, nullptr, (int)T, (int)F, (int)B, (int)S, split};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");

@@ -50,6 +50,7 @@ struct MfmaLaunch {
     const uint8_t* x = nullptr;
     const float* w = nullptr;
     float* out = nullptr;
+    const uint8_t* mask = nullptr;  // [T][F] frame validity (0 = lost), or null
     int T = 0, F = 0, B = 0, S = 0;
     bool split = true;
 };
@@ -113,12 +114,12 @@ __device__ __forceinline__ void load_weights(const float2* __restrict__ w, int F
 
 // At least two waves per SIMD (at most 256 registers per lane): the
 // unrolled stage otherwise keeps several tiles' accumulators in flight.
-template <int A, int NT, int WV, bool SPLIT, int MODE, int ST, bool DB>
+template <int A, int NT, int WV, bool SPLIT, int MODE, int ST, bool DB, bool MASKED = false>
 __global__ __launch_bounds__(WV * kWaveSize) __attribute__((amdgpu_waves_per_eu(2)))
 void mfma_fused_kernel(
     const uint8_t* __restrict__ x, const float2* __restrict__ w, float* __restrict__ out,
-    int T, int F, int B, int S, int W, int tiles_total, int tiles_per_chunk, int n_items,
-    int per_xcd) {
+    const uint8_t* __restrict__ mask, int T, int F, int B, int S, int W, int tiles_total,
+    int tiles_per_chunk, int n_items, int per_xcd) {
     constexpr int kThreads = WV * kWaveSize;
     constexpr int KS = A / 16;                      // k-steps of 32 (16 antennas x re/im)
     using Map = StageMap<A, kThreads>;
@@ -191,14 +192,17 @@ void mfma_fused_kernel(
     };
     unsigned pre[DW_THREAD];
     // Global load of dword k of the stage whose first spectrum is t_stage;
-    // GUARD: rows at or after t_lim read as zero.
+    // GUARD: rows at or after t_lim, and with MASKED rows of lost frames
+    // (mask byte 0), read as zero.  MASKED kernels always take this path.
     auto load_dword = [&](auto guard, int k, int t_stage, int t_lim) {
         constexpr bool GUARD = decltype(guard)::value;
         const int rel = t_stage + tile_rows(tl0 + Map::tld(k));
         const uint8_t* p = xs + (size_t)rel * row_stride + Map::byte_delta(k);
         unsigned v = 0u;
         if (GUARD) {
-            if (dword_valid(k) && prow + rel < t_lim) v = *reinterpret_cast<const unsigned*>(p);
+            const int t = prow + rel;
+            if (dword_valid(k) && t < t_lim && (!MASKED || mask[(size_t)t * F + f] != 0))
+                v = *reinterpret_cast<const unsigned*>(p);
         } else if (dword_valid(k)) {
             v = *reinterpret_cast<const unsigned*>(p);
         }
@@ -213,7 +217,7 @@ void mfma_fused_kernel(
     };
     auto load_stage = [&](int st0) {
         const int ts = stage_time(st0);
-        if (stage_full(st0)) {
+        if (!MASKED && stage_full(st0)) {
 #pragma unroll
             for (int k = 0; k < DW_THREAD; ++k) load_dword(std::false_type{}, k, ts, T);
         } else {
@@ -409,7 +413,7 @@ void mfma_fused_kernel(
             // lies before T; otherwise guarded (t_lim 0 when it does not exist).
             const int st2 = st0 + 2 * ST;
             const int t_load = stage_time(st2);
-            if (st2 < tile_end && stage_full(st2)) {
+            if (!MASKED && st2 < tile_end && stage_full(st2)) {
                 if (has_beams)
                     stage(std::true_type{}, std::false_type{}, cur, nxt, st0, t_load, T);
                 else
@@ -491,15 +495,16 @@ inline Plan make_plan(int T, int F, int B, int S, const MfmaConfig& cfg) {
     return p;
 }
 
-template <int A, int NT, int WV, bool SPLIT, bool DB>
+template <int A, int NT, int WV, bool SPLIT, bool DB, bool MASKED = false>
 void launch_shape(hipStream_t stream, const MfmaLaunch& L, const MfmaConfig& cfg) {
     const Plan p = make_plan(L.T, L.F, L.B, L.S, cfg);
     const dim3 grid((unsigned)(p.per_xcd * kXcds), (unsigned)p.beam_blocks);
     const dim3 block(WV * kWaveSize);
     const float2* w2 = reinterpret_cast<const float2*>(L.w);
 #define BF_LAUNCH(MODE_, ST_)                                                                 \
-    hipLaunchKernelGGL((mfma_fused_kernel<A, NT, WV, SPLIT, MODE_, ST_, DB>), grid, block, 0, \
-                       stream, L.x, w2, L.out, L.T, L.F, L.B, L.S, p.W, p.tiles_total,         \
+    hipLaunchKernelGGL((mfma_fused_kernel<A, NT, WV, SPLIT, MODE_, ST_, DB, MASKED>), grid,   \
+                       block, 0,                                                               \
+                       stream, L.x, w2, L.out, L.mask, L.T, L.F, L.B, L.S, p.W, p.tiles_total, \
                        p.tiles_per_chunk, p.n_items, p.per_xcd)
     if (p.mode == 0) {
         BF_LAUNCH(0, 4);
@@ -539,8 +544,21 @@ void launch_split_db(hipStream_t stream, const MfmaLaunch& L, const MfmaConfig& 
 #define BF_MFMA_SHAPES_128 BF_SHAPE(128, 2, 4) BF_SHAPE(128, 1, 8) BF_SHAPE(128, 1, 4)
 #define BF_MFMA_SHAPES_256 BF_SHAPE(256, 1, 4)
 
+// A launch with a loss mask runs the one masked instantiation per antenna
+// count: 8 beams per wave, 4 waves, one LDS stage (any beam count).
 #define BF_DEFINE_MFMA_FOR_A(A_, SHAPES)                                                   \
     bool launch_mfma_a##A_(hipStream_t stream, const MfmaLaunch& L, const MfmaConfig& cfg) { \
+        if (L.mask != nullptr) {                                                           \
+            MfmaConfig m = cfg;                                                            \
+            m.nt = 1;                                                                      \
+            m.waves = 4;                                                                   \
+            m.db = 0;                                                                      \
+            if (L.split)                                                                   \
+                mfma_detail::launch_shape<A_, 1, 4, true, false, true>(stream, L, m);      \
+            else                                                                           \
+                mfma_detail::launch_shape<A_, 1, 4, false, false, true>(stream, L, m);     \
+            return true;                                                                   \
+        }                                                                                  \
         SHAPES                                                                             \
         return false;                                                                      \
     }

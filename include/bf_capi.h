@@ -120,6 +120,23 @@ bf_status bf_launch_quantize_integrated_intensity(void* stream,
                                                   const bf_dims* integrated_dims);
 
 /* Which kernel AUTO resolves to for these dimensions (a bf_kernel value). */
+/* The stage launchers with a loss mask: one validity byte per [t][f] frame
+ * (device memory, 0 = lost; include/beamformer/formats.hpp:22-32).  A lost
+ * frame's spectrum contributes nothing: its per-spectrum power is 0 and
+ * integrated windows sum the valid spectra only.  The payload is read, never
+ * rewritten.  The reference defines the mask but never applies it; no
+ * reference launcher corresponds.  MFMA kernels only (SPLIT_F16, F16, or
+ * AUTO where it selects one); other selectors return INVALID_ARGUMENT. */
+bf_status bf_launch_packed_beamformer_masked(int kernel, void* stream, const uint8_t* packed,
+                                             const uint8_t* loss_mask, const float* weights,
+                                             float* intensity, const bf_dims* dims);
+bf_status bf_launch_packed_integrated_beamformer_masked(int kernel, void* stream,
+                                                        const uint8_t* packed,
+                                                        const uint8_t* loss_mask,
+                                                        const float* weights, float* integrated,
+                                                        const bf_dims* dims,
+                                                        size_t integration_spectra);
+
 bf_status bf_resolve_kernel(int kernel, const bf_dims* dims, size_t integration_spectra,
                             int* resolved);
 
