@@ -66,6 +66,7 @@ struct ServeWork {
 
 struct ServeLaunch {
     const uint8_t* ring = nullptr;
+    const uint8_t* valid = nullptr;  // [R][F] frame-validity ring, or null (all valid)
     long long ring_mask = 0;
     size_t F = 0, A = 0;
     Geom geo{};

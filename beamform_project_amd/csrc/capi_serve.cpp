@@ -9,6 +9,11 @@
 // and every step batches the windows that completed since the previous step
 // -- across all active requests -- into one captured launch.
 //
+// Spectra reach the ring by copy (ingest, on an ingest stream) or are
+// written in place by the caller's producer (acquire / commit on the
+// producer's own stream); with loss masks enabled a validity ring [R][F]
+// rides along and lost frames contribute nothing to any window.
+//
 // Threading: one host thread drives a server (ingest, step, read, release);
 // the ring copy runs on an ingest stream and the steps on a step stream.
 #include <hip/hip_runtime.h>
@@ -160,6 +165,7 @@ struct bf_server {
     bool use_graph = false;
     hipStream_t step_stream = nullptr, ingest_stream = nullptr;
     uint8_t* d_ring = nullptr;
+    uint8_t* d_valid = nullptr;  // [R][F] validity bytes when loss masks are on
     unsigned char* d_arena = nullptr;
     ServeSlot* d_slots = nullptr;
     ServeCtrl* d_ctrl = nullptr;
@@ -182,6 +188,7 @@ struct bf_server {
     long long head = 0;       // spectra ingested
     long long processed = 0;  // head released to the last issued step
     long long done_head = 0;  // head of the last harvested step
+    size_t acquired = 0;      // spectra handed to a producer by acquire, not committed
     std::deque<Commit> commits;
     std::deque<Step> steps;
     std::vector<hipEvent_t> event_pool;
@@ -207,7 +214,7 @@ struct bf_server {
         for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (void* p : {(void*)d_ring, (void*)d_arena, (void*)d_slots, (void*)d_ctrl,
+        for (void* p : {(void*)d_ring, (void*)d_valid, (void*)d_arena, (void*)d_slots, (void*)d_ctrl,
                         (void*)d_work, (void*)d_pos, (void*)d_freqs})
             if (p) (void)hipFree(p);
         if (h_staging) (void)hipHostFree(h_staging);
@@ -395,9 +402,16 @@ struct bf_server {
                 n_windows += hi - lo;
             }
         }
-        const Commit* c = commit_of(H - 1);
-        if (c != nullptr)
-            hip_check(hipStreamWaitEvent(step_stream, c->ev, 0), "hipStreamWaitEvent");
+        // Every commit landing spectra in [processed, H): ingest copies are
+        // ordered on one stream, but producer commits come from streams of
+        // the caller's choosing.  Earlier spectra were covered by earlier
+        // steps, which precede this one on the step stream.
+        {
+            auto it = std::upper_bound(commits.begin(), commits.end(), processed,
+                                       [](long long v, const Commit& c) { return v < c.h1; });
+            for (; it != commits.end() && it->h0 < H; ++it)
+                hip_check(hipStreamWaitEvent(step_stream, it->ev, 0), "hipStreamWaitEvent");
+        }
         // Bound the steps in flight so a head-ring entry is never reused
         // before its step has read it.
         while (steps.size() >= (size_t)kServeHeadRing / 2) {
@@ -426,46 +440,100 @@ struct bf_server {
         return n_windows;
     }
 
-    void ingest(const uint8_t* src, size_t n, bool on_device) {
-        if (n == 0) return;
+    // The step that must finish before spectra [head, head + n) may be
+    // written: spectra below X = head + n - R are overwritten, so every
+    // window holding one must have been processed, i.e. a step with
+    // H >= X + max_window.  Null when that has already been harvested.
+    const Step* overrun_guard(size_t n) const {
         if (n + cfg.max_window > R)
-            invalid("an ingest of " + std::to_string(n) +
-                    " spectra exceeds the ring capacity less the longest window");
-        // Spectra below X are overwritten: every window holding one must
-        // have been processed, i.e. a step with H >= X + max_window.
+            invalid("writing " + std::to_string(n) +
+                    " spectra at once exceeds the ring capacity less the longest window");
         const long long X = head + (long long)n - (long long)R;
-        if (X > 0) {
-            const long long need = X + (long long)cfg.max_window;
-            if (need > done_head) {
-                const Step* guard = nullptr;
-                for (const Step& s : steps)
-                    if (s.H >= need) {
-                        guard = &s;
-                        break;
-                    }
-                if (guard == nullptr)
-                    fail(BF_ERR_LOGIC,
-                         "ring overrun: step the server before ingesting more spectra");
-                hip_check(hipStreamWaitEvent(ingest_stream, guard->ev_done, 0),
-                          "hipStreamWaitEvent");
-            }
+        if (X <= 0) return nullptr;
+        const long long need = X + (long long)cfg.max_window;
+        if (need <= done_head) return nullptr;
+        for (const Step& s : steps)
+            if (s.H >= need) return &s;
+        fail(BF_ERR_LOGIC, "ring overrun: step the server before ingesting more spectra");
+    }
+
+    // Copies `rows` rows of `row` bytes into ring rows [at, at + rows),
+    // split at the wrap; a null source fills with `fill`.
+    void ring_copy(uint8_t* base, size_t row, size_t at, const uint8_t* src, size_t rows,
+                   hipMemcpyKind kind, unsigned char fill) {
+        const size_t first = std::min(rows, R - at);
+        const size_t parts[2][2] = {{at, first}, {0, rows - first}};
+        size_t done = 0;
+        for (const auto& pr : parts) {
+            if (pr[1] == 0) continue;
+            if (src != nullptr)
+                hip_check(hipMemcpyAsync(base + pr[0] * row, src + done * row, pr[1] * row, kind,
+                                         ingest_stream),
+                          "hipMemcpyAsync ring");
+            else
+                hip_check(hipMemsetAsync(base + pr[0] * row, fill, pr[1] * row, ingest_stream),
+                          "hipMemsetAsync ring");
+            done += pr[1];
         }
-        const size_t row = F * A;
+    }
+
+    void ingest(const uint8_t* src, const uint8_t* mask, size_t n, bool on_device) {
+        if (acquired != 0)
+            fail(BF_ERR_LOGIC, "an acquired ring span must be committed before ingesting");
+        if (mask != nullptr && d_valid == nullptr)
+            invalid("loss masks need a server created with loss_masks = 1");
+        if (n == 0) return;
+        const Step* guard = overrun_guard(n);
+        if (guard != nullptr)
+            hip_check(hipStreamWaitEvent(ingest_stream, guard->ev_done, 0), "hipStreamWaitEvent");
         const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
         const size_t at = (size_t)(head & (long long)(R - 1));
-        const size_t first = std::min(n, R - at);
-        hip_check(hipMemcpyAsync(d_ring + at * row, src, first * row, kind, ingest_stream),
-                  "hipMemcpyAsync ring");
-        if (first < n)
-            hip_check(hipMemcpyAsync(d_ring, src + first * row, (n - first) * row, kind,
-                                     ingest_stream),
-                      "hipMemcpyAsync ring");
+        ring_copy(d_ring, F * A, at, src, n, kind, 0);
+        // Spectra ingested without a mask are valid in every channel.
+        if (d_valid != nullptr) ring_copy(d_valid, F, at, mask, n, kind, 1);
         Commit c{head, head + (long long)n, get_event()};
         hip_check(hipEventRecord(c.ev, ingest_stream), "hipEventRecord");
         commits.push_back(c);
         head += (long long)n;
         // A host source may be reused by the caller as soon as this returns.
         if (!on_device) hip_check(hipEventSynchronize(c.ev), "hipEventSynchronize");
+    }
+
+    // Hands the producer the ring rows of spectra [head, head + n): one span,
+    // or two when they wrap.  Blocks the host until no step still reads
+    // them.  Spectra become visible to steps at commit.
+    void acquire(size_t n, bf_ring_span spans[2], size_t* n_spans) {
+        if (acquired != 0) fail(BF_ERR_LOGIC, "a ring span is already acquired");
+        if (n == 0) invalid("acquire needs at least one spectrum");
+        const Step* guard = overrun_guard(n);
+        if (guard != nullptr) hip_check(hipEventSynchronize(guard->ev_done), "hipEventSynchronize");
+        const size_t at = (size_t)(head & (long long)(R - 1));
+        const size_t first = std::min(n, R - at);
+        *n_spans = first < n ? 2 : 1;
+        const size_t rows[2] = {first, n - first};
+        const size_t starts[2] = {at, 0};
+        long long t = head;
+        for (size_t i = 0; i < *n_spans; ++i) {
+            spans[i].first_spectrum = t;
+            spans[i].n_spectra = rows[i];
+            spans[i].voltages = d_ring + starts[i] * F * A;
+            spans[i].valid = d_valid ? d_valid + starts[i] * F : nullptr;
+            t += (long long)rows[i];
+        }
+        acquired = n;
+    }
+
+    // Publishes the first n acquired spectra once the producer's work
+    // queued so far on `stream` has finished.
+    void commit(size_t n, hipStream_t stream) {
+        if (acquired == 0) fail(BF_ERR_LOGIC, "commit without an acquired ring span");
+        if (n > acquired) invalid("commit of more spectra than were acquired");
+        acquired = 0;
+        if (n == 0) return;
+        Commit c{head, head + (long long)n, get_event()};
+        hip_check(hipEventRecord(c.ev, stream), "hipEventRecord");
+        commits.push_back(c);
+        head += (long long)n;
     }
 
     void release(uint64_t id) {
@@ -559,6 +627,10 @@ void create_server(const bf_server_config* c, std::unique_ptr<bf_server>* out) {
     const size_t ring_bytes = R * s->F * s->A;
     hip_check(hipMalloc(reinterpret_cast<void**>(&s->d_ring), ring_bytes), "hipMalloc ring");
     hip_check(hipMemset(s->d_ring, 0, ring_bytes), "hipMemset ring");
+    if (c->loss_masks != 0) {
+        hip_check(hipMalloc(reinterpret_cast<void**>(&s->d_valid), R * s->F), "hipMalloc valid");
+        hip_check(hipMemset(s->d_valid, 1, R * s->F), "hipMemset valid");
+    }
     hip_check(hipMalloc(reinterpret_cast<void**>(&s->d_arena), c->hbm_budget_bytes),
               "hipMalloc arena");
     hip_check(hipMalloc(reinterpret_cast<void**>(&s->d_slots),
@@ -613,6 +685,7 @@ void create_server(const bf_server_config* c, std::unique_ptr<bf_server>* out) {
     s->slot_owner.assign(c->max_requests, 0);
     ServeLaunch& L = s->launch;
     L.ring = s->d_ring;
+    L.valid = s->d_valid;
     L.ring_mask = (long long)(R - 1);
     L.F = s->F;
     L.A = s->A;
@@ -692,7 +765,34 @@ bf_status bf_server_ingest(bf_server* s, const uint8_t* src, size_t n_spectra, i
     return guarded([&] {
         require_ptr(s, "server");
         require_ptr(src, "src");
-        s->ingest(src, n_spectra, on_device != 0);
+        s->ingest(src, nullptr, n_spectra, on_device != 0);
+    });
+}
+
+bf_status bf_server_ingest_masked(bf_server* s, const uint8_t* src, const uint8_t* valid,
+                                  size_t n_spectra, int on_device) {
+    return guarded([&] {
+        require_ptr(s, "server");
+        require_ptr(src, "src");
+        require_ptr(valid, "valid");
+        s->ingest(src, valid, n_spectra, on_device != 0);
+    });
+}
+
+bf_status bf_server_acquire(bf_server* s, size_t n_spectra, bf_ring_span* spans,
+                            size_t* n_spans) {
+    return guarded([&] {
+        require_ptr(s, "server");
+        require_ptr(spans, "spans");
+        require_ptr(n_spans, "n_spans");
+        s->acquire(n_spectra, spans, n_spans);
+    });
+}
+
+bf_status bf_server_commit(bf_server* s, size_t n_spectra, void* producer_stream) {
+    return guarded([&] {
+        require_ptr(s, "server");
+        s->commit(n_spectra, static_cast<hipStream_t>(producer_stream));
     });
 }
 
@@ -867,7 +967,7 @@ bf_status bf_server_simulate(const bf_server_sim_config* c, bf_server_sim_report
                 next_arrival += gap(rng);
             }
             const size_t at = (size_t)((s->head) % (long long)pool);
-            s->ingest(d_pool + at * row, chunk, true);
+            s->ingest(d_pool + at * row, nullptr, chunk, true);
             while (s->processed < s->head) s->step();
             // Release finished requests: their output memory returns to the
             // arena and their slots to the table.

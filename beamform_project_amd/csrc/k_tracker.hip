@@ -308,12 +308,16 @@ __device__ __forceinline__ void window_operands(const DirArgs& da, const float* 
 // x_f + (t & ring_mask) * row_stride: a linear [T][F][A] array uses
 // ring_mask = -1, a ring of 2^k spectra uses 2^k - 1.  Window w covers
 // spectra [t_base + w*S, t_base + (w+1)*S) clipped to t_limit.
+// With a loss mask, frame (t, f) is valid when valid_f[(t & ring_mask) * F]
+// is non-zero; a lost frame's row reads as zero.
 struct RowSource {
     const uint8_t* x_f;  // x + f * A
     size_t row_stride;   // F * A bytes
     long ring_mask;
     long t_base;
     long t_limit;
+    const uint8_t* valid_f = nullptr;  // mask + f, [t][F] bytes (MASKED only)
+    size_t mask_stride = 0;             // F
 };
 
 // Output of one channel: integrated [slot][F][B] with slot = w mod
@@ -346,7 +350,8 @@ struct TileRef {
 // window fits one tile and fewer than kWaves blocks hold beams, the idle
 // waves share the blocks and the tiles of each stage are dealt out among a
 // block's waves, so every wave generates weights.
-template <int A, int NT, bool SPLIT, bool INTEGRATE, int DIRSRC, int ST, bool ANCHORED>
+template <int A, int NT, bool SPLIT, bool INTEGRATE, int DIRSRC, int ST, bool ANCHORED,
+          bool MASKED = false>
 __device__ __forceinline__ void tracker_windows(const RowSource& rs, const float* s_pos,
                                                 const DirArgs& da, const OutTarget& ot, double k,
                                                 int w_begin, int w_end, int tpw, int wg_beam0) {
@@ -416,7 +421,7 @@ __device__ __forceinline__ void tracker_windows(const RowSource& rs, const float
     // after the first, and no row needs a guard.
     auto load_stage = [&](int q0, unsigned (&dst)[DW_THREAD]) {
         const long t0 = rs.t_base + (long)(w_begin + lw) * S + 16 * lj;
-        const bool fast = lj + ST <= tpw && 16 * (lj + ST) <= S && q0 + ST <= q_end &&
+        const bool fast = !MASKED && lj + ST <= tpw && 16 * (lj + ST) <= S && q0 + ST <= q_end &&
                           t0 + 16 * ST <= rs.t_limit &&
                           (mask < 0 || (t0 & mask) <= mask + 1 - 16 * ST);
         if (fast) {
@@ -434,8 +439,9 @@ __device__ __forceinline__ void tracker_windows(const RowSource& rs, const float
                 unsigned v = 0u;
                 if (dword_valid(kk) && rho < r.rows) {
                     const size_t row = (size_t)((r.t + rho) & mask);
-                    v = *reinterpret_cast<const unsigned*>(x_thread + row * row_stride +
-                                                           Map::byte_delta(kk));
+                    if (!MASKED || rs.valid_f[row * rs.mask_stride] != 0)
+                        v = *reinterpret_cast<const unsigned*>(x_thread + row * row_stride +
+                                                               Map::byte_delta(kk));
                 }
                 dst[kk] = v;
             }
@@ -783,8 +789,9 @@ __device__ __forceinline__ int owner_slot(const int* off, int i) {
 // weights, MFMA order, window sum and cross-lane fold -- is that of
 // tracker_windows for one-tile windows, so results are bit-identical to the
 // multi-beam tracker.
-template <int A, bool SPLIT>
+template <int A, bool SPLIT, bool MASKED>
 __device__ __forceinline__ void serve_short_task(const uint8_t* __restrict__ ring,
+                                                 const uint8_t* __restrict__ valid,
                                                  long long ring_mask, int F,
                                                  const float* s_pos, const ServeSlot* sl,
                                                  int f, int beam0, int w_begin, int w_end,
@@ -811,9 +818,13 @@ __device__ __forceinline__ void serve_short_task(const uint8_t* __restrict__ rin
     unsigned cur[KS], nxt[KS];
 #pragma unroll
     for (int s = 0; s < KS; ++s) cur[s] = nxt[s] = 0u;
+    // frame (t, f) of the ring is lost when its validity byte is 0
+    auto frame_ok = [&](long t) {
+        return !MASKED || valid[(size_t)(t & ring_mask) * F + f] != 0;
+    };
     {
         const long t = t0 + (long)w_begin * S + row;
-        if (row < S && t < te) {
+        if (row < S && t < te && frame_ok(t)) {
             const uint8_t* p = base + (size_t)(t & ring_mask) * row_stride;
 #pragma unroll
             for (int s = 0; s < KS; ++s) cur[s] = *reinterpret_cast<const unsigned*>(p + 4 * s);
@@ -823,7 +834,7 @@ __device__ __forceinline__ void serve_short_task(const uint8_t* __restrict__ rin
     for (int w = w_begin; w < w_end; ++w) {
         {
             const long t = t0 + (long)(w + 1) * S + row;
-            const bool ok = w + 1 < w_end && row < S && t < te;
+            const bool ok = w + 1 < w_end && row < S && t < te && frame_ok(t);
             const uint8_t* p = base + (size_t)(t & ring_mask) * row_stride;
 #pragma unroll
             for (int s = 0; s < KS; ++s)
@@ -877,11 +888,11 @@ __device__ __forceinline__ void serve_short_task(const uint8_t* __restrict__ rin
 // multi-tile-window slots (four waves cover up to 64 beams), then each wave
 // takes wave tasks of one-tile-window slots on its own.
 // Two workgroups per CU (the persistent grid) need at most 256 registers.
-template <int A, bool SPLIT>
+template <int A, bool SPLIT, bool MASKED>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
 void serve_items_kernel(
-    const uint8_t* __restrict__ ring, long long ring_mask, int F, Geom geo,
-    const ServeSlot* __restrict__ slots, const ServeWork* __restrict__ work) {
+    const uint8_t* __restrict__ ring, const uint8_t* __restrict__ valid, long long ring_mask,
+    int F, Geom geo, const ServeSlot* __restrict__ slots, const ServeWork* __restrict__ work) {
     __shared__ int off_l[kServeMaxSlots + 1];
     __shared__ int off_s[kServeMaxSlots + 1];
     __shared__ float s_pos[3 * A];
@@ -911,10 +922,10 @@ void serve_items_kernel(
         const int w_end = w_begin + sl->wpi < w_last ? w_begin + sl->wpi : w_last;
         const DirArgs da{nullptr, &sl->traj[0][0], 0L, S, sl->B};
         const RowSource rs{ring + (size_t)f * A, (size_t)F * A, (long)ring_mask, (long)sl->t0,
-                           (long)sl->t_end};
+                           (long)sl->t_end, MASKED ? valid + f : nullptr, (size_t)F};
         const OutTarget ot{sl->out, (long)sl->out_windows, F, f};
         const double k = wave_number(channel_frequency(geo, f));
-        tracker_windows<A, 2, SPLIT, true, kDirTrajectory, 4, true>(
+        tracker_windows<A, 2, SPLIT, true, kDirTrajectory, 4, true, MASKED>(
             rs, s_pos, da, ot, k, w_begin, w_end, (S + 15) / 16, 0);
     }
     const int wave = threadIdx.x / kWaveSize;
@@ -933,7 +944,8 @@ void serve_items_kernel(
         const int w_end =
             w_begin + kServeShortRun < w_last ? w_begin + kServeShortRun : w_last;
         const double k = wave_number(channel_frequency(geo, f));
-        serve_short_task<A, SPLIT>(ring, ring_mask, F, s_pos, sl, f, block * 16, w_begin, w_end,
+        serve_short_task<A, SPLIT, MASKED>(ring, valid, ring_mask, F, s_pos, sl, f, block * 16,
+                                           w_begin, w_end,
                                    k);
     }
 }
@@ -1043,9 +1055,17 @@ void launch_tracker(hipStream_t stream, const TrackerLaunch& L) {
 void launch_serve_step(hipStream_t stream, const ServeLaunch& L) {
     hipLaunchKernelGGL(serve_plan_kernel, dim3(1), dim3(kServeMaxSlots), 0, stream, L.slots,
                        L.ctrl, L.work, L.heads, (int)L.F);
-#define BF_SERVE(A_, SPLIT_)                                                                 \
-    hipLaunchKernelGGL((serve_items_kernel<A_, SPLIT_>), dim3((unsigned)L.grid), dim3(kThreads), \
-                       0, stream, L.ring, L.ring_mask, (int)L.F, L.geo, L.slots, L.work)
+#define BF_SERVE(A_, SPLIT_)                                                                  \
+    do {                                                                                      \
+        if (L.valid != nullptr)                                                               \
+            hipLaunchKernelGGL((serve_items_kernel<A_, SPLIT_, true>), dim3((unsigned)L.grid), \
+                               dim3(kThreads), 0, stream, L.ring, L.valid, L.ring_mask,       \
+                               (int)L.F, L.geo, L.slots, L.work);                             \
+        else                                                                                  \
+            hipLaunchKernelGGL((serve_items_kernel<A_, SPLIT_, false>),                       \
+                               dim3((unsigned)L.grid), dim3(kThreads), 0, stream, L.ring,     \
+                               L.valid, L.ring_mask, (int)L.F, L.geo, L.slots, L.work);       \
+    } while (0)
     if (L.A == 32 && L.split)
         BF_SERVE(32, true);
     else if (L.A == 32)

@@ -24,7 +24,12 @@ class ServerConfigC(C.Structure):
     _fields_ = [("n_freq", _sz), ("n_ant", _sz), ("ring_spectra", _sz), ("max_window", _sz),
                 ("max_step_spectra", _sz), ("hbm_budget_bytes", _sz), ("max_requests", _sz),
                 ("positions", _vp), ("frequencies", _vp), ("kernel", C.c_int),
-                ("enable_graph", C.c_int)]
+                ("enable_graph", C.c_int), ("loss_masks", C.c_int)]
+
+
+class RingSpanC(C.Structure):
+    _fields_ = [("first_spectrum", C.c_longlong), ("n_spectra", _sz), ("voltages", _vp),
+                ("valid", _vp)]
 
 
 class RequestC(C.Structure):
@@ -81,6 +86,9 @@ for _name, _args in {
     "bf_server_destroy": [_vp],
     "bf_server_submit": [_vp, C.POINTER(RequestC), C.POINTER(C.c_uint64)],
     "bf_server_ingest": [_vp, _vp, _sz, C.c_int],
+    "bf_server_ingest_masked": [_vp, _vp, _vp, _sz, C.c_int],
+    "bf_server_acquire": [_vp, _sz, C.POINTER(RingSpanC), C.POINTER(_sz)],
+    "bf_server_commit": [_vp, _sz, _vp],
     "bf_server_step": [_vp, C.POINTER(_sz)],
     "bf_server_request_status": [_vp, C.c_uint64, C.POINTER(RequestStatusC)],
     "bf_server_read_output": [_vp, C.c_uint64, _sz, _sz, _vp],
@@ -108,6 +116,7 @@ class ServerConfig:
     frequencies: Optional[np.ndarray] = None   # [F] Hz
     kernel: Kernel = Kernel.SPLIT_F16
     enable_graph: bool = True
+    loss_masks: bool = False                   # validity ring [ring][F]
 
     def to_c(self, keep: list) -> ServerConfigC:
         pos = fr = None
@@ -121,7 +130,7 @@ class ServerConfig:
                              self.max_step_spectra, self.hbm_budget_bytes, self.max_requests,
                              None if pos is None else pos.ctypes.data,
                              None if fr is None else fr.ctypes.data, int(self.kernel),
-                             int(self.enable_graph))
+                             int(self.enable_graph), int(self.loss_masks))
 
 
 @dataclass
@@ -164,15 +173,44 @@ class TrackerServer:
         call("bf_server_submit", self._h, C.byref(q), C.byref(rid))
         return rid.value
 
-    def ingest(self, voltages) -> None:
-        """Append spectra [n][F][A]: a host uint8 array or an api.DeviceArray."""
+    def ingest(self, voltages, valid=None) -> None:
+        """Append spectra [n][F][A]: a host uint8 array or an api.DeviceArray,
+        with optional validity bytes [n][F] of the same kind (0 = lost frame;
+        needs a server with loss_masks)."""
+        row = self.cfg.n_freq * self.cfg.n_ant
         if isinstance(voltages, np.ndarray):
             v = np.ascontiguousarray(voltages, np.uint8)
-            n = v.size // (self.cfg.n_freq * self.cfg.n_ant)
-            call("bf_server_ingest", self._h, v.ctypes.data, n, 0)
+            n, on_dev, src = v.size // row, 0, v.ctypes.data
         else:
-            n = voltages.nbytes // (self.cfg.n_freq * self.cfg.n_ant)
-            call("bf_server_ingest", self._h, voltages.ptr, n, 1)
+            n, on_dev, src = voltages.nbytes // row, 1, voltages.ptr
+        if valid is None:
+            call("bf_server_ingest", self._h, src, n, on_dev)
+            return
+        if on_dev:
+            if valid.nbytes != n * self.cfg.n_freq:
+                raise ValueError("valid must hold [n_spectra][n_freq] bytes")
+            call("bf_server_ingest_masked", self._h, src, valid.ptr, n, 1)
+        else:
+            m = np.ascontiguousarray(valid, np.uint8)
+            if m.size != n * self.cfg.n_freq:
+                raise ValueError("valid must hold [n_spectra][n_freq] bytes")
+            call("bf_server_ingest_masked", self._h, src, m.ctypes.data, n, 0)
+
+    def acquire(self, n_spectra: int) -> list:
+        """Ring rows for the next n_spectra spectra, written in place by the
+        caller: a list of one or two dicts (first_spectrum, n_spectra,
+        voltages, valid) with device addresses."""
+        spans = (RingSpanC * 2)()
+        n = _sz()
+        call("bf_server_acquire", self._h, n_spectra, spans, C.byref(n))
+        return [{"first_spectrum": spans[i].first_spectrum, "n_spectra": spans[i].n_spectra,
+                 "voltages": spans[i].voltages or 0, "valid": spans[i].valid or 0}
+                for i in range(n.value)]
+
+    def commit(self, n_spectra: int, stream: int = 0) -> None:
+        """Publishes acquired spectra once the work queued on `stream`
+        (a hipStream_t address, 0 = null stream) has finished."""
+        call("bf_server_commit", self._h, n_spectra, stream or None)
 
     def ingest_device(self, ptr: int, n_spectra: int) -> None:
         call("bf_server_ingest", self._h, ptr, n_spectra, 1)

@@ -461,7 +461,14 @@ bf_status bf_doa_set_state(bf_doa* doa, const double* state, int initialised);
  * (hipGraph when enable_graph).  Per-window latency runs from the ingest
  * copy that landed a window's last spectrum to the end of the step that
  * produced it (device events).  One host thread drives a server; calls
- * that overlap on one server race. */
+ * that overlap on one server race.
+ *
+ * With loss_masks set, the server keeps one validity byte per frame
+ * (include/beamformer/formats.hpp:22-32: [t][F], 0 = lost) in a second
+ * ring [ring_spectra][F]; a lost frame contributes nothing to any window
+ * (bit for bit the result of zeroing its voltages).  Spectra may also be
+ * written in place by the caller's producer (bf_server_acquire /
+ * bf_server_commit) instead of copied in by bf_server_ingest. */
 typedef struct bf_server bf_server;
 
 typedef struct bf_server_config {
@@ -476,7 +483,18 @@ typedef struct bf_server_config {
     const float* frequencies; /* host [n_freq] Hz, copied; NULL: reference plan */
     int kernel;               /* SPLIT_F16, F16 or AUTO (= SPLIT_F16) */
     int enable_graph;
+    int loss_masks;           /* 1: keep a validity ring and honour loss masks */
 } bf_server_config;
+
+/* Ring rows of consecutive spectra handed to a producer: voltages
+ * [n_spectra][F][A] and, with loss masks, validity bytes [n_spectra][F]
+ * (NULL without).  Device pointers. */
+typedef struct bf_ring_span {
+    long long first_spectrum;
+    size_t n_spectra;
+    uint8_t* voltages;
+    uint8_t* valid;
+} bf_ring_span;
 
 typedef struct bf_server_request {
     size_t n_beams;              /* 1..64 */
@@ -539,6 +557,24 @@ bf_status bf_server_submit(bf_server* server, const bf_server_request* request,
  * overwrite spectra of windows no step has processed yet. */
 bf_status bf_server_ingest(bf_server* server, const uint8_t* src, size_t n_spectra,
                            int src_on_device);
+/* As bf_server_ingest, with the spectra's validity bytes valid [n][F]
+ * (same memory kind as src); needs a server with loss_masks.  Spectra
+ * ingested without a mask are valid in every channel. */
+bf_status bf_server_ingest_masked(bf_server* server, const uint8_t* src, const uint8_t* valid,
+                                  size_t n_spectra, int src_on_device);
+/* Zero-copy producer interface: hands out the ring rows of the next
+ * n_spectra spectra (one span, two when they wrap; spans[2]) for the
+ * caller to write on a stream of its own.  Blocks the host until no step
+ * still reads those rows; BF_ERR_LOGIC when no step launched so far
+ * would release them.  With loss masks the producer writes the validity
+ * bytes of every spectrum too.  Until bf_server_commit, no other ingest
+ * or acquire is accepted. */
+bf_status bf_server_acquire(bf_server* server, size_t n_spectra, bf_ring_span* spans,
+                            size_t* n_spans);
+/* Publishes the first n_spectra acquired spectra (0 abandons the span):
+ * steps wait for the work queued on producer_stream (a hipStream_t; NULL
+ * is the null stream) up to this call. */
+bf_status bf_server_commit(bf_server* server, size_t n_spectra, void* producer_stream);
 /* Admits what fits, then enqueues one step over the spectra ingested since
  * the previous step (at most max_step_spectra); returns the windows it
  * scheduled.  Asynchronous. */

@@ -4,6 +4,9 @@ Bars: a request's integrated output [W][F][B] is bit-identical whether it
 runs alone or mixed with other requests, and to the multi-beam tracker
 (same kernel arithmetic) over the same span of voltages; against the FP64
 oracle it holds the split-FP16 bar (floored relative error <= 1e-5).
+A loss-masked stream equals, bit for bit, the unmasked stream with its lost
+frames zeroed (include/beamformer/formats.hpp:22-32), and spectra written
+in place through acquire/commit equal the same spectra ingested by copy.
 The ring is kept small so that windows wrap around it.
 """
 import numpy as np
@@ -52,17 +55,40 @@ MIX = [
 ]
 
 
-def run(requests, x, chunk=100, **kw):
-    """Streams x through a server, submitting request i once `submit_at`
-    spectra have been ingested; returns {i: (start, output)}."""
-    srv = SV.TrackerServer(config(**kw))
+def produce(srv, x, mask, producer):
+    """Writes x (and mask) into the ring in place: acquire, copy each span,
+    commit on the producer's stream."""
+    spans = srv.acquire(x.shape[0])
+    i = 0
+    for sp in spans:
+        n = sp["n_spectra"]
+        assert sp["first_spectrum"] == srv.stats()["head"] + i
+        part = np.ascontiguousarray(x[i:i + n])
+        _capi.call("bf_memcpy_htod", sp["voltages"], part.ctypes.data, part.nbytes)
+        if mask is not None:
+            m = np.ascontiguousarray(mask[i:i + n])
+            _capi.call("bf_memcpy_htod", sp["valid"], m.ctypes.data, m.nbytes)
+        i += n
+    srv.commit(x.shape[0], producer.handle)
+
+
+def run(requests, x, chunk=100, mask=None, zero_copy=False, **kw):
+    """Streams x (with validity bytes `mask`) through a server, submitting
+    request i once `submit_at` spectra have been ingested; returns
+    {i: (start, output)}."""
+    srv = SV.TrackerServer(config(loss_masks=mask is not None, **kw))
+    producer = api.Stream() if zero_copy else None
     ids = {}
     T = x.shape[0]
     for h in range(0, T, chunk):
         for i, (B, S, n, start, at, seed) in enumerate(requests):
             if at == h:
                 ids[i] = srv.submit(SV.Request(trajectories(B, seed), S, n, start))
-        srv.ingest(x[h:h + chunk])
+        m = None if mask is None else mask[h:h + chunk]
+        if zero_copy:
+            produce(srv, x[h:h + chunk], m, producer)
+        else:
+            srv.ingest(x[h:h + chunk], m)
         srv.step()
     srv.synchronize()
     out = {}
@@ -240,3 +266,61 @@ def test_native_load_generator_reports_latency():
     assert 0 < st["latency_p50_ms"] <= st["latency_p99_ms"] <= st["latency_max_ms"]
     assert st["beam_samples"] > 0 and rep["beam_samples_per_s"] > 0
     assert rep["graph_used"] == 1
+
+
+def loss_mask(T, seed):
+    rng = np.random.default_rng(seed)
+    mask = (rng.random((T, F)) > 0.2).astype(np.uint8)
+    mask[450:530] = 0     # a run of lost spectra across every channel
+    mask[:, 5] = 0        # a dead channel
+    return mask
+
+
+def test_masked_stream_equals_zeroed_payload(stream, mixed):
+    mask = loss_mask(stream.shape[0], 3)
+    zeroed = stream * mask[:, :, None]
+    got, _ = run(MIX, stream, mask=mask)
+    want, _ = run(MIX, zeroed)
+    for i in want:
+        assert got[i][0] == want[i][0]
+        assert np.array_equal(got[i][1], want[i][1]), f"request {i}"
+        assert np.all(got[i][1][:, 5, :] == 0)
+    # a masked server fed every frame valid equals the unmasked server
+    ones, _ = run(MIX, stream, mask=np.ones(mask.shape, np.uint8), enable_graph=False)
+    for i in ones:
+        assert np.array_equal(ones[i][1], mixed[0][i][1]), f"request {i}"
+
+
+def test_zero_copy_producer_equals_ingest(stream, mixed):
+    # chunks of 100 on a ring of 1024: spans wrap at 1000..1100 and 2000..2100
+    got, _ = run(MIX, stream, zero_copy=True)
+    for i in got:
+        assert np.array_equal(got[i][1], mixed[0][i][1]), f"request {i}"
+    mask = loss_mask(stream.shape[0], 4)
+    masked, _ = run(MIX, stream, mask=mask, zero_copy=True)
+    zeroed, _ = run(MIX, stream * mask[:, :, None])
+    for i in zeroed:
+        assert np.array_equal(masked[i][1], zeroed[i][1]), f"request {i}"
+
+
+def test_producer_interface_contract(stream):
+    srv = SV.TrackerServer(config(ring_spectra=1024))
+    with pytest.raises(_capi.InvalidArgument, match="loss_masks"):
+        srv.ingest(stream[:10], np.ones((10, F), np.uint8))
+    with pytest.raises(_capi.LogicError, match="without an acquired"):
+        srv.commit(10)
+    spans = srv.acquire(100)
+    assert len(spans) == 1 and spans[0]["valid"] == 0 and spans[0]["n_spectra"] == 100
+    with pytest.raises(_capi.LogicError, match="already acquired"):
+        srv.acquire(10)
+    with pytest.raises(_capi.LogicError, match="committed before ingesting"):
+        srv.ingest(stream[:10])
+    with pytest.raises(_capi.InvalidArgument, match="more spectra"):
+        srv.commit(101)
+    srv.commit(60)             # a partial commit publishes the first 60
+    assert srv.stats()["head"] == 60
+    assert srv.acquire(600)[0]["first_spectrum"] == 60
+    srv.commit(0)              # abandoned
+    with pytest.raises(_capi.InvalidArgument, match="ring capacity"):
+        srv.acquire(1024 - 320 + 1)
+    srv.close()
