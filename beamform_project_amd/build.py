@@ -42,6 +42,7 @@ COMMON_FLAGS = [
 PER_FILE_FLAGS = {
     "k_quant.hip": ["-ffp-contract=off"],
     "k_tracker.hip": ["-ffp-contract=off"],
+    "k_opt_tracker.hip": ["-ffp-contract=off"],
 }
 
 

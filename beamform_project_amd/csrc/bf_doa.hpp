@@ -22,4 +22,43 @@ void launch_doa_estimate(hipStream_t stream, const double* capon, const double* 
                          size_t F, size_t n_theta, const double* theta, size_t max_peaks,
                          double* broadband, double* peaks, int* n_peaks);
 
+// ----------------------------------------------- planar direction search
+// Device state of one search: the centre of the window's coarse grid (the
+// previous window's estimate) and the running centre of the refinement.
+struct OptSearch {
+    float prev[3];
+    float centre[3];
+};
+
+struct OptGridArgs {
+    int G;         // coarse cells per axis
+    int M;         // sub-array size (steered antennas 0 .. M-1)
+    float fov_l;   // half-extent of the coarse grid
+    float fov_m;
+    int coplanar;  // steer at nominal lattice points (every |z| <= 1e-6)
+    int interp;    // quadratic peak interpolation
+};
+
+// Per channel: R_block = Rs / (K L) (forward-backward averaged when fb),
+// diagonal load (Capon), forgetting fold into state [F][M][M][2] FP64, and
+// the scan operand scanmat [F][M][M]: the strict upper triangle of R
+// (Bartlett, trace in trace[F]) or the transposed inverse Cholesky factor
+// (Capon); mode[F] = 0 Bartlett form, 1 Capon form.
+void launch_opt_fold(hipStream_t stream, const int* rs, double* state, bool first, size_t F,
+                     size_t M, size_t K, size_t L, bool fb, bool capon, float load,
+                     float lambda, float2* scanmat, float* trace, int* mode);
+
+// part[n_cells][F]: each channel's power at the cells of `level` (0: the
+// G x G grid around search->prev, else the 3 x 3 refinement candidates).
+void launch_opt_scan(hipStream_t stream, const float2* scanmat, const float* trace,
+                     const int* mode, const float* pos, const float* freqs,
+                     const OptSearch* search, const OptGridArgs& g, int level, size_t F,
+                     size_t n_cells, float* part);
+
+// Sums part over channels and takes the level's decision; at last_level
+// writes the window's direction to dirs[window][3].
+void launch_opt_decide(hipStream_t stream, const float* part, size_t F, size_t n_cells,
+                       const OptGridArgs& g, int level, int last_level, size_t window,
+                       OptSearch* search, float* dirs);
+
 }  // namespace bfhip

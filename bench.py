@@ -42,7 +42,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "b1"])
+    p.add_argument("--workload", default="c2",
+                   choices=["c2", "c1", "c3", "c4", "c5", "b1", "d2"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-serving", action="store_true")
@@ -112,9 +113,9 @@ def main():
         api.device_synchronize()
         print(json.dumps({"calibration": "3 x 1 GiB device-to-device copies"}), flush=True)
         return
-    if args.workload in ("c3", "b1", "c4", "c5"):
+    if args.workload in ("c3", "b1", "c4", "c5", "d2"):
         line = {"c3": bench_c3, "b1": bench_b1, "c4": bench_c4,
-                "c5": bench_c5}[args.workload](args)
+                "c5": bench_c5, "d2": bench_d2}[args.workload](args)
         print(json.dumps(line), flush=True)
         return
     wl = workload(args.workload)
@@ -499,6 +500,89 @@ def bench_c5(args):
                   "kernel": "covariance", "kernel_avg_ms": kt["covariance_ms"],
                   "algorithmic_bytes": cov_bytes},
         cpu_baseline=cpu)
+
+
+def bench_d2(args):
+    """Adaptive planar DOA tracker (CpuOptBeamTracker's searching path) on
+    the reference benchmark's configuration
+    (benchmarks/benchmark_cpu_opt_beam_tracker.cpp:56-84): A=64, F=336,
+    T=15360, S=320 (48 windows), 12 x 12 coarse grid over +-0.2, 2
+    refinement levels, prior at zenith, source near l=0.03.  One step is one
+    run over the shard: the 48 window searches back to back, then the
+    emission; per-frame time = step time / windows."""
+    from beamform_project_amd import api, opt_tracker as OT
+    A, F_, T, S, G, LV = 64, 336, 15360, 320, 12, 2
+    W = T // S
+    stream = api.Stream()
+    x = api.DeviceArray.empty((T, F_, A), np.uint8)
+    pos = np.zeros((A, 3), np.float32)
+    pos[:, 0] = (np.arange(A) % 8) * np.float32(0.6)
+    pos[:, 1] = (np.arange(A) // 8) * np.float32(0.6)
+    fr = (np.float32(300e6) + np.arange(F_, dtype=np.float32) * np.float32(300e3))
+    dpos, dfr = api.DeviceArray.from_numpy(pos), api.DeviceArray.from_numpy(fr)
+    src = np.array([0.03, 0.0, np.sqrt(1 - 0.03 ** 2)], np.float32)
+    api.generate_point_source_noise(stream, x, T, F_, A, dpos, dfr, src, 4.0, 1.0, 2024)
+    out = api.DeviceArray.empty((T, F_, 1), np.float32)
+    stream.synchronize()
+    dims = api.Dimensions(T, F_, A, 1)
+    res, dirs = {}, {}
+    for name, est in (("bartlett", OT.TrackerEstimator.BARTLETT),
+                      ("capon", OT.TrackerEstimator.CAPON)):
+        cfg = OT.OptTrackerConfig(est, G, LV, 0.2, 0.2, 0, 1.0, 1e-3, True, S)
+        t = OT.OptBeamTracker(pos, fr, dims, cfg)
+        t.seed_trajectory((0.0, 0.0, 1.0))
+
+        def launch():
+            t.run_device(x, out, stream)
+        ms, kms = timed_launches(stream, launch, args.steps, args.warmup)
+        d = t.window_directions()
+        dirs[name] = [round(float(d[-1, 0]), 5), round(float(d[-1, 1]), 5)]
+        t.close()
+        res[name] = {"ms_per_run": round(ms, 4), "gpu_ms_per_run": round(kms, 4),
+                     "ms_per_frame": round(kms / W, 4), "last_window_lm": dirs[name]}
+    ms = res["bartlett"]["gpu_ms_per_run"]
+    cpu = None if args.no_cpu_baseline else opt_tracker_cpu_baseline(args.cpu_seconds)
+    return common_line(
+        args, "windows/s (adaptive DOA tracker frames)", W / (ms * 1e-3), "windows/s", ms,
+        "int8 MFMA covariance (int32 exact), f64 fold/Cholesky, f32 scan, split-f16 emission",
+        {"workload": "adaptive planar DOA tracker: A=64 (reference grid 0.6 m), F=336, "
+                     "T=15360, S=320 (48 frames), 12x12 coarse grid +-0.2, 2 refinement "
+                     "levels, Bartlett; point source at l=0.03 in noise",
+         "n_ant": A, "n_freq": F_, "n_time": T, "integration_spectra": S,
+         "coarse_grid_resolution": G, "refinement_levels": LV},
+        ms_per_frame=res["bartlett"]["ms_per_frame"], target_ms_per_frame=0.5, paths=res,
+        cpu_baseline=cpu)
+
+
+def opt_tracker_cpu_baseline(seconds):
+    """The reference's CpuOptBeamTracker (oracle/_ref, OpenMP over the host's
+    cores) on a bounded sample of the same configuration: whole frames of
+    S=320 spectra, scaled to windows/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker/baseline library, never the product path
+    lib = O.ref_lib()
+    if lib is None:
+        return None
+    A, F_, S = 64, 336, 320
+    pos, fr = O.default_positions(A), O.channelized_frequencies(F_)
+
+    def run(n_frames):
+        T = n_frames * S
+        x = O.make_moving_point_source(T, F_, A, pos, fr, 0.03, 0.0, 1e-5, 0.0)
+        t0 = time.perf_counter()
+        O.ref_opt_tracker(lib, x, 1, T, F_, A, pos, fr, (0.0, 0.0, 1.0), (0.0, 0.0), S, 0, 12,
+                          2, 0.2, 0.2, 0, 1.0, 1e-3, True)
+        return time.perf_counter() - t0
+
+    dt = run(1)
+    n = int(max(1, min(48, seconds / 2 / max(dt, 1e-6))))
+    med = float(np.median([run(n) for _ in range(2)]))
+    return {"value": round(n / med, 2), "unit": "windows/s",
+            "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+            "kind": "reference",
+            "sample": f"{n} frames of S=320 (A=64, F=336, G=12, 2 levels, Bartlett) through "
+                      f"the reference's CpuOptBeamTracker; median of 2 runs of {med:.2f} s, "
+                      f"{1e3 * med / n:.1f} ms per frame"}
 
 
 def doa_cpu_baseline(seconds, A, K, M, NTH):

@@ -440,6 +440,73 @@ bf_status bf_doa_last_timings(bf_doa* doa, bf_doa_timings* timings);
 bf_status bf_doa_get_state(bf_doa* doa, double* state, int* initialised);
 bf_status bf_doa_set_state(bf_doa* doa, const double* state, int initialised);
 
+/* ---------------------------------------- adaptive planar DOA tracker */
+/* GPU counterpart of the reference's CpuOptBeamTracker
+ * (include/beamformer/cpu_opt_beam_tracker.hpp:39-196,
+ * src/cpu_opt_beam_tracker.cpp:1084-2494): per integration window, the
+ * per-channel covariance recursion, a G x G coarse (l, m) scan around the
+ * previous estimate summed over channels, refinement_levels 3 x 3
+ * halvings around the running best, quadratic peak interpolation, and the
+ * window's intensity [T][F][1] along the estimate.  With
+ * coarse_grid_resolution <= 1 the scan is off and each window follows the
+ * seeded trajectory (the open-loop tracker).  The covariance state
+ * persists across runs (reset clears it); the first window of every run is
+ * searched around the trajectory's window-0 direction.  A = 32 or 64;
+ * coarse_grid_resolution <= 64. */
+enum bf_estimator { BF_ESTIMATOR_BARTLETT = 0, BF_ESTIMATOR_CAPON = 1 };
+
+/* Reference: CpuOptTrackerConfig (cpu_opt_beam_tracker.hpp:50-88); `kernel`
+ * (FP32, SPLIT_F16, F16 or AUTO = SPLIT_F16) computes the emitted
+ * intensities. */
+typedef struct bf_opt_tracker_config {
+    int estimator;
+    size_t coarse_grid_resolution;
+    size_t refinement_levels;
+    float search_fov_l;
+    float search_fov_m;
+    size_t spatial_smoothing_subarray_size;
+    float forgetting_factor;
+    float diagonal_load;
+    int enable_quadratic_peak_interp;
+    size_t integration_spectra;
+    int kernel;
+} bf_opt_tracker_config;
+
+typedef struct bf_opt_tracker bf_opt_tracker;
+/* The reference's default-constructed CpuOptTrackerConfig. */
+bf_status bf_opt_tracker_default_config(bf_opt_tracker_config* config);
+/* Reference: CpuOptBeamTracker::CpuOptBeamTracker (.cpp:1084-1320).
+ * positions [n_ant][3] m and frequencies [n_freq] Hz are host arrays,
+ * copied; NULL selects default_positions / channelized_frequencies. */
+bf_status bf_opt_tracker_create(const float* positions, const float* frequencies,
+                                const bf_dims* dims, const bf_opt_tracker_config* config,
+                                bf_opt_tracker** out);
+bf_status bf_opt_tracker_destroy(bf_opt_tracker* tracker);
+/* Reference: seed_trajectory (.cpp:1335-1357). */
+bf_status bf_opt_tracker_seed_trajectory(bf_opt_tracker* tracker, const float* direction_start,
+                                         const float* direction_rate_per_sample);
+/* Clears the covariance recursion (the next window starts it afresh). */
+bf_status bf_opt_tracker_reset(bf_opt_tracker* tracker);
+/* Reference: run_into (.cpp:1619-2457) on device buffers, packed
+ * [T][F][A] -> intensity [T][F][1]; asynchronous on `stream`. */
+bf_status bf_opt_tracker_run_device(bf_opt_tracker* tracker, void* stream,
+                                    const uint8_t* d_packed, float* d_intensity);
+/* run_into on host buffers (size checks of .cpp:1626-1633). */
+bf_status bf_opt_tracker_run(bf_opt_tracker* tracker, const uint8_t* packed,
+                             size_t packed_count, float* intensity, size_t intensity_count);
+/* Reference: window_direction (.cpp:1328-1333) for every window of the last
+ * run, [n][3]; dirs may be NULL to query the count. */
+bf_status bf_opt_tracker_window_directions(bf_opt_tracker* tracker, float* dirs,
+                                           size_t capacity, size_t* n_windows);
+/* Reference: cpu_opt_beam_tracker_packed_intensity_into, stateless
+ * (.cpp:2471-2486): default geometry, a fresh recursion, the trajectory's
+ * integration_spectra. */
+bf_status bf_opt_tracker_packed_intensity(const uint8_t* packed, size_t packed_count,
+                                          const bf_dims* dims,
+                                          const bf_tracker_config* trajectory,
+                                          const bf_opt_tracker_config* config, float* intensity,
+                                          size_t intensity_count);
+
 /* ------------------------------------------- streaming tracker server */
 /* Continuous-batching server for tracking requests over one voltage stream
  * (addition of this build; a superset of the reference's

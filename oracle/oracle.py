@@ -347,9 +347,209 @@ def ref_lib():
         "ref_read_packed_shard": [C.c_char_p, C.c_char_p, C.c_char_p, _sz, _sz, _sz, _sz,
                                   _u8p, _u8p, C.POINTER(C.c_uint64)],
         "ref_write_tiled_weights_file": [C.c_char_p, _f32p, _sz, _sz, _sz, _sz],
+        "ref_cpu_opt_tracker": [_u8p, _sz, _sz, _sz, _sz, _f32p, _f32p, _f32p, _f32p, _sz,
+                                C.c_int, _sz, _sz, C.c_float, C.c_float, _sz, C.c_float,
+                                C.c_float, C.c_int, _f32p, _f32p],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)
         fn.restype = C.c_int
         fn.argtypes = args
     return lib
+
+
+def ref_opt_tracker(lib, x, n_calls, T, F, A, pos, freqs, start, rate, S, estimator=0, G=1,
+                    levels=0, fov_l=0.2, fov_m=0.2, p_sub=0, lam=1.0, load=1e-3, interp=True):
+    """The reference's CpuOptBeamTracker (src/cpu_opt_beam_tracker.cpp:1619-2457)
+    over n_calls consecutive chunks of T spectra: (intensity [n][T][F],
+    directions [n][W][3])."""
+    W = (T + S - 1) // S
+    out = np.zeros((n_calls, T, F), np.float32)
+    dirs = np.zeros((n_calls, W, 3), np.float32)
+    _check(lib.ref_cpu_opt_tracker(np.ascontiguousarray(x, np.uint8).reshape(-1), n_calls, T, F,
+                                   A, np.ascontiguousarray(pos, np.float32),
+                                   np.ascontiguousarray(freqs, np.float32),
+                                   np.ascontiguousarray(start, np.float32),
+                                   np.ascontiguousarray(rate, np.float32), S, int(estimator), G,
+                                   levels, fov_l, fov_m, p_sub, lam, load, int(interp), out,
+                                   dirs), "ref_cpu_opt_tracker")
+    return out, dirs
+
+
+# ------------------------------------------ adaptive planar DOA search
+# Restatement of CpuOptBeamTracker::run_into's searching path
+# (src/cpu_opt_beam_tracker.cpp:1619-2457) in FP64 numpy, with the
+# reference's float32 grid and decision arithmetic.  fb_in_place=True
+# reproduces the reference's forward-backward step, which reads entries it
+# has already averaged (:647-656); False is the textbook average
+# (R + J conj(R) J) / 2 of :486-498.
+def _f32(v):
+    return np.float32(v)
+
+
+def _clamp_dir(l, m):
+    """Horizon clamp and direction_from_lm (:2350-2357, src/geometry.cpp:97-106)."""
+    l, m = _f32(l), _f32(m)
+    r2 = l * l + m * m
+    if r2 > _f32(0.999):
+        s = np.sqrt(_f32(0.999) / r2)
+        l, m = l * s, m * s
+    return np.array([l, m, np.sqrt(_f32(1.0) - (l * l + m * m))], np.float32)
+
+
+def _coarse_cells(c, G, fov_l, fov_m, coplanar):
+    """build_centred_grid (:798-828) and the steered directions (:1971-1993)."""
+    fl, fm = _f32(fov_l), _f32(fov_m)
+    step_l = _f32(0.0) if G == 1 else (_f32(2.0) * fl) / _f32(G - 1)
+    step_m = _f32(0.0) if G == 1 else (_f32(2.0) * fm) / _f32(G - 1)
+    sl = 0.0 if G == 1 else (2.0 * float(fl)) / (G - 1)
+    sm = 0.0 if G == 1 else (2.0 * float(fm)) / (G - 1)
+    mid = (G - 1) * 0.5
+    dirs, steer = [], []
+    for v in range(G):
+        for u in range(G):
+            d = _clamp_dir((c[0] - fl) + _f32(u) * step_l, (c[1] - fm) + _f32(v) * step_m)
+            st = d.astype(np.float64)
+            if coplanar:
+                nl = float(c[0]) + (u - mid) * sl
+                nm = float(c[1]) + (v - mid) * sm
+                if not (nl * nl + nm * nm > 0.999):
+                    st = np.array([nl, nm, 0.0])
+            dirs.append(d)
+            steer.append(st)
+    return np.array(dirs), np.array(steer)
+
+
+def _refine_cells(c, G, fov_l, fov_m, level, coplanar):
+    """3 x 3 candidates of refinement `level` (:2179-2233)."""
+    hl, hm = _f32(fov_l) / _f32(G), _f32(fov_m) / _f32(G)
+    for _ in range(level):
+        hl, hm = hl * _f32(0.5), hm * _f32(0.5)
+    dirs, steer = [], []
+    for dv in (-1, 0, 1):
+        for du in (-1, 0, 1):
+            l, m = c[0] + _f32(du) * hl, c[1] + _f32(dv) * hm
+            clamped = l * l + m * m > _f32(0.999)
+            d = _clamp_dir(l, m)
+            if coplanar and not clamped:
+                st = np.array([float(c[0]) + du * float(hl), float(c[1]) + dv * float(hm), 0.0])
+            else:
+                st = d.astype(np.float64)
+            dirs.append(d)
+            steer.append(st)
+    return np.array(dirs), np.array(steer), hl, hm
+
+
+def _quadratic_interp(pc, p_up, p_down, p_vp, p_vm):
+    """quadratic_interp (:830-853), float32."""
+    pc, p_up, p_down, p_vp, p_vm = map(_f32, (pc, p_up, p_down, p_vp, p_vm))
+    du = dv = _f32(0.0)
+    den = p_down - _f32(2.0) * pc + p_up
+    if den > 0:
+        du = np.clip((_f32(0.5) * (p_up - p_down)) / den, _f32(-0.5), _f32(0.5))
+    den = p_vm - _f32(2.0) * pc + p_vp
+    if den > 0:
+        dv = np.clip((_f32(0.5) * (p_vp - p_vm)) / den, _f32(-0.5), _f32(0.5))
+    return _f32(du), _f32(dv)
+
+
+def _decode(x):
+    """int4 pairs -> complex128 (low nibble real, high nibble imaginary)."""
+    x = np.asarray(x, np.uint8).astype(np.int16)
+    return (((x & 15) ^ 8) - 8) + 1j * ((((x >> 4) & 15) ^ 8) - 8)
+
+
+def opt_tracker_search(x, S, pos, freqs, start, rate, estimator=0, G=16, levels=0,
+                       fov_l=0.2, fov_m=0.2, p_sub=0, lam=1.0, load=1e-3, interp=True,
+                       state=None, fb_in_place=False):
+    """Window directions [W][3] of one run_into call over x [T][F][A], and
+    the covariance state {"R": [F][M][M] complex} carried to the next call
+    (pass it back as `state`)."""
+    x = np.asarray(x)
+    T, F, A = x.shape
+    M = p_sub if (p_sub != 0 and p_sub < A) else A
+    L = A - M + 1
+    smoothing = M < A
+    capon = estimator == 1
+    pos = np.asarray(pos, np.float64)
+    coplanar = bool(np.all(np.abs(np.asarray(pos, np.float32)[:M, 2]) <= np.float32(1e-6)))
+    k = 2.0 * np.pi * np.asarray(freqs, np.float32).astype(np.float64) / 299792458.0
+    state = {} if state is None else state
+    W = (T + S - 1) // S
+    prior = _clamp_dir(start[0], start[1])  # window 0 of the trajectory
+    prev = prior
+    dirs = np.zeros((W, 3), np.float32)
+    lam32 = _f32(lam)
+    for w in range(W):
+        xs = _decode(x[w * S:min(T, w * S + S)])  # [K][F][A]
+        K = xs.shape[0]
+        Rf = np.einsum("kfa,kfb->fab", xs, np.conj(xs))  # [F][A][A]
+        Rs = sum(Rf[:, l:l + M, l:l + M] for l in range(L)) / (K * L)
+        if smoothing:
+            if fb_in_place:
+                for f in range(F):
+                    R = Rs[f]
+                    for r in range(M):
+                        for c in range(M):
+                            R[r, c] = 0.5 * (R[r, c] + np.conj(R[M - 1 - r, M - 1 - c]))
+            else:
+                Rs = 0.5 * (Rs + np.conj(Rs[:, ::-1, ::-1]))
+        if capon and load > 0:
+            tr = np.trace(Rs, axis1=1, axis2=2).real
+            Rs = Rs + (float(load) * tr / M)[:, None, None] * np.eye(M)
+        if "R" not in state or lam32 >= 1:
+            state["R"] = Rs
+        else:
+            state["R"] = float(lam32) * state["R"] + float(_f32(1.0) - lam32) * Rs
+        R = state["R"]
+        chol = []
+        if capon:
+            for f in range(F):
+                try:
+                    Lf = np.linalg.cholesky(R[f])
+                    ok = bool(np.all(np.diag(Lf).real ** 2 > 1e-12))
+                except np.linalg.LinAlgError:
+                    ok = False
+                chol.append(np.linalg.inv(Lf) if ok else None)
+
+        def powers(steer):
+            ph = k[:, None, None] * (steer[None, :, :] @ pos[:M].T)  # [F][cells][M]
+            a = np.exp(-1j * ph)
+            if not capon:
+                p = np.einsum("fca,fab,fcb->fc", np.conj(a), R, a, optimize=True).real
+            else:
+                p = np.zeros(a.shape[:2])
+                for f in range(F):
+                    if chol[f] is None:
+                        p[f] = np.einsum("ca,ab,cb->c", np.conj(a[f]), R[f], a[f]).real
+                    else:
+                        v = a[f] @ chol[f].T
+                        n2 = np.sum(np.abs(v) ** 2, axis=1)
+                        p[f] = np.where(n2 > 1e-12, 1.0 / n2, 0.0)
+            p[~np.isfinite(p)] = 0.0
+            return p.sum(axis=0).astype(np.float32)
+
+        cd, cs = _coarse_cells(prev, G, fov_l, fov_m, coplanar)
+        P = powers(cs)
+        best = int(np.argmax(P))
+        centre = cd[best]
+        if interp and levels == 0:
+            bu, bv = best % G, best // G
+
+            def at(u, v):
+                uu, vv = bu + u, bv + v
+                return P[vv * G + uu] if 0 <= uu < G and 0 <= vv < G else 0.0
+            du, dv = _quadratic_interp(P[best], at(1, 0), at(-1, 0), at(0, 1), at(0, -1))
+            step_l = _f32(0.0) if G == 1 else (_f32(2.0) * _f32(fov_l)) / _f32(G - 1)
+            step_m = _f32(0.0) if G == 1 else (_f32(2.0) * _f32(fov_m)) / _f32(G - 1)
+            centre = _clamp_dir(centre[0] + du * step_l, centre[1] + dv * step_m)
+        for level in range(1, levels + 1):
+            rd, rs, hl, hm = _refine_cells(centre, G, fov_l, fov_m, level, coplanar)
+            P = powers(rs)
+            centre = rd[int(np.argmax(P))]
+            if interp:
+                du, dv = _quadratic_interp(P[4], P[5], P[3], P[7], P[1])
+                centre = _clamp_dir(centre[0] + du * hl, centre[1] + dv * hm)
+        prev = centre
+        dirs[w] = centre
+    return dirs, state

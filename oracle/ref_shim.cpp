@@ -7,6 +7,7 @@
 #include "beamformer/beam_tracker.hpp"
 #include "beamformer/beam_tracker_opt_v2.hpp"
 #include "beamformer/cpu_beamformer.hpp"
+#include "beamformer/cpu_opt_beam_tracker.hpp"
 #include "beamformer/geometry.hpp"
 #include "beamformer/io.hpp"
 #include "beamformer/quantization.hpp"
@@ -281,6 +282,48 @@ int ref_write_tiled_weights_file(const char* path, const float* w, std::size_t T
         TiledWeights tw(n);
         std::memcpy(tw.data(), w, n * sizeof(ComplexFloat));
         write_tiled_weights(path, tw, d);
+    });
+}
+
+// n_calls consecutive run_into calls of one CpuOptBeamTracker over
+// consecutive T-spectrum chunks of x (the covariance recursion carries
+// over), each seeded with the same prior; out [n_calls][T][F], dirs
+// [n_calls][W][3].
+int ref_cpu_opt_tracker(const std::uint8_t* x, std::size_t n_calls, std::size_t T, std::size_t F,
+                        std::size_t A, const float* pos, const float* freqs, const float* start,
+                        const float* rate, std::size_t S, int estimator, std::size_t G,
+                        std::size_t levels, float fov_l, float fov_m, std::size_t p_sub,
+                        float lambda, float load, int interp, float* out, float* dirs) {
+    return guarded([&] {
+        const Dimensions dims{T, F, A, 1};
+        CpuOptTrackerConfig c;
+        c.estimator = estimator ? TrackerEstimator::Capon : TrackerEstimator::Bartlett;
+        c.coarse_grid_resolution = G;
+        c.refinement_levels = levels;
+        c.search_fov_l = fov_l;
+        c.search_fov_m = fov_m;
+        c.spatial_smoothing_subarray_size = p_sub;
+        c.forgetting_factor = lambda;
+        c.diagonal_load = load;
+        c.enable_quadratic_peak_interp = interp != 0;
+        c.integration_spectra = S;
+        CpuOptBeamTracker tracker(vec3s(pos, A), std::vector<float>(freqs, freqs + F), dims, c);
+        TrackerTrajectoryConfig prior;
+        prior.direction_start = Vec3{start[0], start[1], start[2]};
+        prior.direction_rate_per_sample = {rate[0], rate[1]};
+        const std::size_t W = tracker_window_count(T, S);
+        const std::size_t chunk = T * F * A;
+        Intensities intensity(T * F);
+        for (std::size_t k = 0; k < n_calls; ++k) {
+            PackedVoltage packed(x + k * chunk, x + (k + 1) * chunk);
+            tracker.seed_trajectory(prior);
+            tracker.run_into(packed, intensity);
+            std::memcpy(out + k * T * F, intensity.data(), T * F * sizeof(float));
+            for (std::size_t w = 0; w < W; ++w) {
+                const Vec3 d = tracker.window_direction(w);
+                for (int i = 0; i < 3; ++i) dirs[(k * W + w) * 3 + i] = d[i];
+            }
+        }
     });
 }
 

@@ -15,6 +15,12 @@ Each case mirrors a case of the reference's own tests:
   tracker_moving_a32    tests/cuda/test_cuda_tracker_v2.cpp:45-58 (shortened)
   io_quantized_meta.txt src/io.cpp:149-183 (write_quantized_output_metadata)
   io_shard_*.bin/.txt   src/io.cpp:244-278 (write_packed_shard, T=2 A=32)
+  opt_tracker_*         tests/cpu/test_cpu_opt_beam_tracker.cpp:290-645 (the
+                        adaptive tracker: Bartlett, Capon, coarse-only with
+                        interpolation, A=64, forgetting on a moving source,
+                        state carried across calls, scan disabled, smoothing);
+                        inputs are regenerated from the stored source and
+                        checked against the stored CRC-32
 
 Run:  python tests/golden/make_golden.py
 """
@@ -22,6 +28,7 @@ from __future__ import annotations
 
 import os
 import sys
+import zlib
 
 import numpy as np
 
@@ -32,6 +39,37 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import oracle as O  # noqa: E402
 
 F = 336
+
+
+# start: the prior's direction_start; rate its direction_rate_per_sample;
+# source: the (moving) point source l0, m0, rate_l, rate_m of the input.
+_Z = (0.0, 0.0, 1.0)
+OPT_CASES = {
+    "bartlett_static": dict(A=32, T=4, S=4, source=(0.05, -0.03, 0.0, 0.0), start=_Z,
+                            rate=(0.0, 0.0), estimator=0, G=16, levels=2, fov=0.2, p_sub=0,
+                            lam=1.0),
+    "capon_static": dict(A=32, T=4, S=4, source=(0.05, -0.03, 0.0, 0.0), start=_Z,
+                         rate=(0.0, 0.0), estimator=1, G=16, levels=2, fov=0.2, p_sub=0,
+                         lam=1.0),
+    "coarse_interp": dict(A=32, T=4, S=4, source=(0.031, 0.017, 0.0, 0.0), start=_Z,
+                          rate=(0.0, 0.0), estimator=0, G=12, levels=0, fov=0.2, p_sub=0,
+                          lam=1.0),
+    "a64_bartlett": dict(A=64, T=4, S=4, source=(-0.04, 0.06, 0.0, 0.0), start=_Z,
+                         rate=(0.0, 0.0), estimator=0, G=16, levels=2, fov=0.2, p_sub=0,
+                         lam=1.0),
+    "moving_forgetting": dict(A=32, T=8, S=2, source=(0.02, -0.01, 5e-3, 2e-3), start=_Z,
+                              rate=(0.0, 0.0), estimator=0, G=16, levels=1, fov=0.3, p_sub=0,
+                              lam=0.9),
+    "capon_stateful": dict(A=32, T=4, S=2, calls=2, source=(0.01, 0.02, 4e-3, -3e-3),
+                           start=(0.01, 0.0, 0.99995), rate=(0.0, 0.0), estimator=1, G=12,
+                           levels=1, fov=0.2, p_sub=0, lam=0.95),
+    "scan_off": dict(A=32, T=8, S=2, source=(0.02, -0.01, 1e-3, 5e-4),
+                     start=(0.02, -0.01, 0.99975), rate=(1e-3, 5e-4), estimator=0, G=1,
+                     levels=0, fov=0.2, p_sub=0, lam=1.0),
+    "smoothing_bartlett": dict(A=32, T=4, S=4, source=(0.05, -0.03, 0.0, 0.0), start=_Z,
+                               rate=(0.0, 0.0), estimator=0, G=16, levels=1, fov=0.2, p_sub=24,
+                               lam=1.0),
+}
 
 
 def main() -> None:
@@ -136,6 +174,23 @@ def main() -> None:
                                     os.path.join(HERE, "io_shard_meta.txt").encode(),
                                     os.path.join(HERE, "io_shard_mask.bin").encode(),
                                     payload, mask, T, F, A, 5, 0, 1234567890123, 2560) == 0
+    # Adaptive DOA tracker (CpuOptBeamTracker::run_into) cases.
+    for name, case in OPT_CASES.items():
+        A, T, n = case["A"], case["T"], case.get("calls", 1)
+        src = case["source"]  # l0, m0, rate_l, rate_m
+        x = np.zeros(n * T * F * A, np.uint8)
+        assert R.ref_make_moving_point_source(n * T, F, A, pos(A), freqs, *src, 4.0, x) == 0
+        out, dirs = O.ref_opt_tracker(R, x, n, T, F, A, pos(A), freqs, case["start"],
+                                      case["rate"], case["S"], case["estimator"], case["G"],
+                                      case["levels"], case["fov"], case["fov"], case["p_sub"],
+                                      case["lam"], 1e-3, True)
+        save("opt_tracker_" + name, A=np.int64(A), T=np.int64(T), calls=np.int64(n),
+             S=np.int64(case["S"]), source=np.array(src, np.float32),
+             start=np.array(case["start"], np.float32), rate=np.array(case["rate"], np.float32),
+             estimator=np.int64(case["estimator"]), G=np.int64(case["G"]),
+             levels=np.int64(case["levels"]), fov=np.float32(case["fov"]),
+             p_sub=np.int64(case["p_sub"]), lam=np.float32(case["lam"]),
+             x_crc=np.int64(zlib.crc32(x.tobytes())), intensity=out, directions=dirs)
     for name in sorted(os.listdir(HERE)):
         if name.endswith(".npz") or name.startswith("io_"):
             print(f"{name:32s} {os.path.getsize(os.path.join(HERE, name)):>9d} bytes")

@@ -7,7 +7,7 @@ import re
 
 import pytest
 
-from beamform_project_amd import _capi, api, doa, serve, tracker  # noqa: F401  (register the ABI)
+from beamform_project_amd import _capi, api, doa, opt_tracker, serve, tracker  # noqa: F401  (register the ABI)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -121,3 +121,27 @@ def test_default_mfma_shapes_are_instantiated():
     tuned = [set(re.findall(r"\{(\d), (\d)\}", l)) for l in lists]
     assert tuned[0] == small and tuned[1] == inst["BF_MFMA_SHAPES_128"]
     assert tuned[2] == inst["BF_MFMA_SHAPES_256"]
+
+
+def test_opt_tracker_rejects_bad_configs():
+    """CpuOptBeamTracker constructor checks (src/cpu_opt_beam_tracker.cpp:1084-1113,
+    tests/cpu/test_cpu_opt_beam_tracker.cpp:163-215), raised before any GPU work."""
+    from beamform_project_amd.opt_tracker import OptBeamTracker, OptTrackerConfig, default_config
+    d = api.Dimensions(4, 336, 32, 1)
+    cases = [
+        (dict(forgetting_factor=0.0), "forgetting_factor"),
+        (dict(forgetting_factor=1.5), "forgetting_factor"),
+        (dict(spatial_smoothing_subarray_size=100), "spatial_smoothing_subarray_size"),
+        (dict(integration_spectra=0), "integration_spectra"),
+        (dict(coarse_grid_resolution=65), "coarse_grid_resolution"),
+    ]
+    for kw, msg in cases:
+        with pytest.raises(_capi.InvalidArgument, match=msg):
+            OptBeamTracker(None, None, d, OptTrackerConfig(**kw))
+    with pytest.raises(_capi.InvalidArgument, match="n_beams == 1"):
+        OptBeamTracker(None, None, api.Dimensions(4, 336, 32, 2), OptTrackerConfig())
+    with pytest.raises(_capi.InvalidArgument, match="32 or 64"):
+        OptBeamTracker(None, None, api.Dimensions(4, 336, 128, 1), OptTrackerConfig())
+    c = default_config()
+    assert (c.coarse_grid_resolution, c.refinement_levels, c.forgetting_factor) == (1, 0, 1.0)
+    assert abs(c.diagonal_load - 1e-3) < 1e-9 and c.enable_quadratic_peak_interp

@@ -6,11 +6,13 @@ library where oracle/_ref was built in this container, and (c) closed forms
 and invariants chosen so that a plausible mistake (dropped term, wrong sign,
 transposed operand) fails at least one of them.
 """
+import os
+
 import numpy as np
 import pytest
 
 import oracle as O
-from helpers import F, golden
+from helpers import F, GOLDEN, golden
 
 A32 = 32
 
@@ -341,3 +343,27 @@ def test_doa_update_recovers_single_source():
         d = O.LIB.or_quadratic_peak_offset(P[i - 1], P[i], P[i + 1])
         est = np.rad2deg(theta[i]) + d * 0.25
         assert abs(est - 17.3) < 0.1
+
+
+def test_opt_tracker_restatement_matches_reference_fixtures():
+    """The numpy restatement of the adaptive search (oracle.opt_tracker_search)
+    against the reference's CpuOptBeamTracker directions, with the
+    reference's in-place forward-backward step for the smoothing case."""
+    import glob
+    import zlib
+    for path in sorted(glob.glob(os.path.join(GOLDEN, "opt_tracker_*.npz"))):
+        c = np.load(path, allow_pickle=False)
+        if int(c["G"]) <= 1:
+            continue
+        A, T, n = int(c["A"]), int(c["T"]), int(c["calls"])
+        pos, fr = O.default_positions(A), O.channelized_frequencies(F)
+        x = O.make_moving_point_source(n * T, F, A, pos, fr, *[float(v) for v in c["source"]])
+        assert zlib.crc32(x.tobytes()) == int(c["x_crc"])
+        x = x.reshape(n, T, F, A)
+        state = None
+        for k in range(n):
+            d, state = O.opt_tracker_search(
+                x[k], int(c["S"]), pos, fr, c["start"], c["rate"], int(c["estimator"]),
+                int(c["G"]), int(c["levels"]), float(c["fov"]), float(c["fov"]),
+                int(c["p_sub"]), float(c["lam"]), 1e-3, True, state, fb_in_place=True)
+            assert np.array_equal(d, c["directions"][k]), (path, k)
