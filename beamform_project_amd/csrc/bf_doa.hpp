@@ -41,19 +41,22 @@ struct OptGridArgs {
 
 // Per channel: R_block = Rs / (K L) (forward-backward averaged when fb),
 // diagonal load (Capon), forgetting fold into state [F][M][M][2] FP64, and
-// the scan operand scanmat [F][M][M]: the strict upper triangle of R
-// (Bartlett, trace in trace[F]) or the transposed inverse Cholesky factor
-// (Capon); mode[F] = 0 Bartlett form, 1 Capon form.
+// the scan operand: Bartlett, the strict upper triangle of R in scanmat
+// [F][M][M] with its trace in trace[F] (mode 0); Capon, R^-1 in scan64
+// [F][M][M] with its trace in trace64[F] (mode 1), or the Bartlett form
+// where the factorisation fails.
 void launch_opt_fold(hipStream_t stream, const int* rs, double* state, bool first, size_t F,
                      size_t M, size_t K, size_t L, bool fb, bool capon, float load,
-                     float lambda, float2* scanmat, float* trace, int* mode);
+                     float lambda, float2* scanmat, float* trace, double2* scan64,
+                     double* trace64, int* mode);
 
 // part[n_cells][F]: each channel's power at the cells of `level` (0: the
 // G x G grid around search->prev, else the 3 x 3 refinement candidates).
 void launch_opt_scan(hipStream_t stream, const float2* scanmat, const float* trace,
-                     const int* mode, const float* pos, const float* freqs,
-                     const OptSearch* search, const OptGridArgs& g, int level, size_t F,
-                     size_t n_cells, float* part);
+                     const double2* scan64, const double* trace64, const int* mode,
+                     const float* pos, const float* freqs, const OptSearch* search,
+                     const OptGridArgs& g, int level, size_t F, size_t n_cells, bool capon,
+                     float* part);
 
 // Sums part over channels and takes the level's decision; at last_level
 // writes the window's direction to dirs[window][3].

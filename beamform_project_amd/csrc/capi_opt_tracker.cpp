@@ -86,6 +86,8 @@ struct bf_opt_tracker {
     double* d_state = nullptr;
     float2* d_scan = nullptr;
     float* d_trace = nullptr;
+    double2* d_scan64 = nullptr;  // Capon: R^-1 per channel
+    double* d_trace64 = nullptr;
     int* d_mode = nullptr;
     float* d_part = nullptr;
     OptSearch* d_search = nullptr;
@@ -96,8 +98,8 @@ struct bf_opt_tracker {
     ~bf_opt_tracker() {
         if (last_stream) (void)hipStreamSynchronize(last_stream);
         for (void* p : {(void*)d_pos, (void*)d_freqs, (void*)d_rs, (void*)d_state, (void*)d_scan,
-                        (void*)d_trace, (void*)d_mode, (void*)d_part, (void*)d_search,
-                        (void*)d_dirs, (void*)d_traj})
+                        (void*)d_trace, (void*)d_scan64, (void*)d_trace64, (void*)d_mode,
+                        (void*)d_part, (void*)d_search, (void*)d_dirs, (void*)d_traj})
             if (p) (void)hipFree(p);
     }
 
@@ -168,12 +170,13 @@ struct bf_opt_tracker {
             const size_t K = std::min(S, T - t0);
             launch_doa_covariance(stream, x, T, F, A, t0, K, M, d_rs, nullptr);
             launch_opt_fold(stream, d_rs, d_state, !initialised, F, M, K, L, smoothing, capon,
-                            cfg.diagonal_load, cfg.forgetting_factor, d_scan, d_trace, d_mode);
+                            cfg.diagonal_load, cfg.forgetting_factor, d_scan, d_trace,
+                            d_scan64, d_trace64, d_mode);
             initialised = true;
             for (int level = 0; level <= last; ++level) {
                 const size_t cells = level == 0 ? G * G : 9;
-                launch_opt_scan(stream, d_scan, d_trace, d_mode, d_pos, d_freqs, d_search, g,
-                                level, F, cells, d_part);
+                launch_opt_scan(stream, d_scan, d_trace, d_scan64, d_trace64, d_mode, d_pos,
+                                d_freqs, d_search, g, level, F, cells, capon, d_part);
                 launch_opt_decide(stream, d_part, F, cells, g, level, last, w, d_search, d_dirs);
             }
         }
@@ -267,6 +270,10 @@ void create_tracker(const float* positions, const float* frequencies, const bf_d
     t->d_state = dmalloc<double>(F * M * M * 2);
     t->d_scan = dmalloc<float2>(F * M * M);
     t->d_trace = dmalloc<float>(F);
+    if (t->capon) {
+        t->d_scan64 = dmalloc<double2>(F * M * M);
+        t->d_trace64 = dmalloc<double>(F);
+    }
     t->d_mode = dmalloc<int>(F);
     t->d_part = dmalloc<float>(F * std::max<size_t>(t->G * t->G, 9));
     t->d_search = dmalloc<OptSearch>(1);

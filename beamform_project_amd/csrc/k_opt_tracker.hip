@@ -11,8 +11,10 @@
 //                   read of already-averaged entries), diagonal load for
 //                   Capon (:244-253, before the fold, :1803-1804), forgetting
 //                   fold R = lambda R + (1 - lambda) R_block (:1880-1892),
-//                   trace; Capon: Cholesky R = L L^H (:398-421) and its
-//                   inverse W = L^-1, so a^H R^-1 a = |W a|^2 (:423-437).
+//                   trace.
+//   opt_factor      Capon only, one wave per channel: Cholesky R = L L^H
+//                   (:398-421) and its inverse W = L^-1, so that
+//                   a^H R^-1 a = |W a|^2 (:423-437).
 //   opt_scan        per (channel, cell): the steering vector
 //                   a_j = exp(-j k p_j . d) (:96-106) and the Bartlett power
 //                   tr(R) + 2 Re sum_{r<c} conj(a_r) R_rc a_c (:287-306) or
@@ -41,7 +43,6 @@ namespace bfhip {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kNc = 8;  // cells per wave pass of the scan
 
 struct cd {
     double re, im;
@@ -161,152 +162,187 @@ __device__ void clamp_direction(float l, float m, float out[3]) {
     out[2] = sqrtf(1.0f - (l * l + m * m));
 }
 
-// ---------------------------------------------------------- fold / factor
-// One workgroup per channel.  LDS: two M x M complex double buffers.
+// ------------------------------------------------------------------ fold
+// One workgroup per channel, one covariance entry per thread and pass:
+// R_block = Rs / (K L), forward-backward averaged when fb (the mirrored
+// entry is read from Rs directly), diagonal load for Capon, and the fold
+// into state.  Bartlett also writes its scan operand and the trace of the
+// folded R; Capon's operand comes from opt_factor_kernel.
 __global__ __launch_bounds__(kThreads) void opt_fold_kernel(
     const int* __restrict__ rs, double* __restrict__ state, int first, int M, int K, int L,
     int fb, int capon, float load, float lambda, float2* __restrict__ scanmat,
     float* __restrict__ trace_out, int* __restrict__ mode_out) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    cd* b0 = reinterpret_cast<cd*>(smem);
-    cd* b1 = b0 + M * M;
+    __shared__ double s_diag[64];
     __shared__ double s_eps;
-    __shared__ int s_ok;
     const int f = blockIdx.x;
     const int tid = threadIdx.x;
     const int MM = M * M;
     const double scale = 1.0 / ((double)K * (double)L);
-    for (int e = tid; e < MM; e += kThreads) {
-        const int* p = rs + ((size_t)f * MM + e) * 2;
-        b0[e] = cd{(double)p[0] * scale, (double)p[1] * scale};
-    }
-    __syncthreads();
-    cd* R = b0;
-    if (fb) {
-        // R = (R + J conj(R) J) / 2
-        for (int e = tid; e < MM; e += kThreads) {
-            const int r = e / M, c = e - (e / M) * M;
-            const cd a = b0[e], b = b0[(M - 1 - r) * M + (M - 1 - c)];
-            b1[e] = cd{0.5 * (a.re + b.re), 0.5 * (a.im - b.im)};
+    const int* rf = rs + (size_t)f * MM * 2;
+    auto block = [&](int r, int c) -> cd {
+        const int e = r * M + c;
+        cd v{(double)rf[2 * e] * scale, (double)rf[2 * e + 1] * scale};
+        if (fb) {
+            // (R + J conj(R) J) / 2
+            const int m = (M - 1 - r) * M + (M - 1 - c);
+            v.re = 0.5 * (v.re + (double)rf[2 * m] * scale);
+            v.im = 0.5 * (v.im - (double)rf[2 * m + 1] * scale);
         }
-        __syncthreads();
-        R = b1;
-    }
+        return v;
+    };
+    if (tid < M) s_diag[tid] = block(tid, tid).re;
+    __syncthreads();
     if (tid == 0) {
         double tr = 0.0;
-        for (int i = 0; i < M; ++i) tr += R[i * M + i].re;
+        for (int i = 0; i < M; ++i) tr += s_diag[i];
         s_eps = (capon && load > 0.0f) ? (double)load * tr / (double)M : 0.0;
-        s_ok = 1;
     }
     __syncthreads();
+    const double eps = s_eps;
     const double lam = (double)lambda;
     const double comp = (double)(1.0f - lambda);
     const bool copy = first || lambda >= 1.0f;
     double* st = state + (size_t)f * MM * 2;
+    float2* Q = scanmat + (size_t)f * MM;
     for (int e = tid; e < MM; e += kThreads) {
-        cd v = R[e];
-        if (e / M == e % M) v.re += s_eps;
+        const int r = e / M, c = e - (e / M) * M;
+        cd v = block(r, c);
+        if (r == c) v.re += eps;
         if (!copy) {
             v.re = lam * st[2 * e] + comp * v.re;
             v.im = lam * st[2 * e + 1] + comp * v.im;
         }
         st[2 * e] = v.re;
         st[2 * e + 1] = v.im;
-        R[e] = v;
-    }
-    __syncthreads();
-    float2* Q = scanmat + (size_t)f * MM;
-    if (!capon) {
+        if (r == c) s_diag[r] = v.re;
         // Bartlett operand: the strict upper triangle, zero elsewhere.
-        for (int e = tid; e < MM; e += kThreads) {
-            const int r = e / M, c = e - (e / M) * M;
-            Q[e] = r < c ? make_float2((float)R[e].re, (float)R[e].im) : make_float2(0.0f, 0.0f);
-        }
-        if (tid == 0) {
-            double tr = 0.0;
-            for (int i = 0; i < M; ++i) tr += R[i * M + i].re;
-            trace_out[f] = (float)tr;
-            mode_out[f] = 0;
-        }
-        return;
+        if (!capon)
+            Q[e] = r < c ? make_float2((float)v.re, (float)v.im) : make_float2(0.0f, 0.0f);
     }
-    // Keep the folded R for a failed factorisation's Bartlett fallback.
-    cd* A = R == b0 ? b1 : b0;
-    for (int e = tid; e < MM; e += kThreads) A[e] = R[e];
+    if (capon) return;
     __syncthreads();
-    // Right-looking Cholesky of A (lower triangle); the pivot floor is the
-    // reference's 1e-12.  Column j is copied to `col` as it is scaled, so
-    // the rank-1 update reads it contiguously (a column of A is strided by
-    // M * 16 bytes, which maps every row to one LDS bank).
-    __shared__ cd col[64];
-    for (int j = 0; j < M; ++j) {
-        if (tid == 0) {
-            const double d = A[j * M + j].re;
-            if (!(d > 1e-12)) s_ok = 0;
-            A[j * M + j] = cd{d > 1e-12 ? sqrt(d) : 1.0, 0.0};
-        }
-        __syncthreads();
-        const double dinv = 1.0 / A[j * M + j].re;
-        for (int i = j + 1 + tid; i < M; i += kThreads) {
-            const cd v = A[i * M + j];
-            const cd w{v.re * dinv, v.im * dinv};
-            A[i * M + j] = w;
-            col[i] = w;
-        }
-        __syncthreads();
-        // A[i][k] -= L[i][j] conj(L[k][j]) for j < k <= i
-        for (int i = j + 1 + (tid >> 4); i < M; i += kThreads / 16) {
-            const cd a = col[i];
-            for (int k = j + 1 + (tid & 15); k <= i; k += 16) {
-                const cd b = col[k];
-                A[i * M + k].re -= a.re * b.re + a.im * b.im;
-                A[i * M + k].im -= a.im * b.re - a.re * b.im;
-            }
-        }
-        __syncthreads();
-    }
-    const int ok = s_ok;
-    if (!ok) {
-        // Bartlett on the loaded, folded covariance (:354-375 fallback).
-        for (int e = tid; e < MM; e += kThreads) {
-            const int r = e / M, c = e - (e / M) * M;
-            Q[e] = r < c ? make_float2((float)R[e].re, (float)R[e].im) : make_float2(0.0f, 0.0f);
-        }
-        if (tid == 0) {
-            double tr = 0.0;
-            for (int i = 0; i < M; ++i) tr += R[i * M + i].re;
-            trace_out[f] = (float)tr;
-            mode_out[f] = 0;
-        }
-        return;
-    }
-    // W = L^-1 (lower), one column per thread, written transposed:
-    // Q[k][i] = W[i][k] so that lane i of the scan reads row k contiguously.
-    cd* Wm = R;  // R is no longer needed
-    for (int j = tid; j < M; j += kThreads) {
-        for (int i = 0; i < M; ++i) {
-            if (i < j) {
-                Wm[i * M + j] = cd{0.0, 0.0};
-                continue;
-            }
-            cd s{i == j ? 1.0 : 0.0, 0.0};
-            for (int k = j; k < i; ++k) {
-                const cd a = A[i * M + k], w = Wm[k * M + j];
-                s.re -= a.re * w.re - a.im * w.im;
-                s.im -= a.re * w.im + a.im * w.re;
-            }
-            const double inv = 1.0 / A[i * M + i].re;
-            Wm[i * M + j] = cd{s.re * inv, s.im * inv};
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < MM; e += kThreads) {
-        const int k = e / M, i = e - (e / M) * M;
-        const cd w = Wm[i * M + k];
-        Q[e] = make_float2((float)w.re, (float)w.im);
-    }
     if (tid == 0) {
-        trace_out[f] = 0.0f;
+        double tr = 0.0;
+        for (int i = 0; i < M; ++i) tr += s_diag[i];
+        trace_out[f] = (float)tr;
+        mode_out[f] = 0;
+    }
+}
+
+// ---------------------------------------------------------------- factor
+// Capon operand: Q = R^-1 of the folded state by in-place Gauss-Jordan
+// elimination in FP64, one workgroup per channel.  Thread t holds the 4 x 4
+// block (rows 4 (t / 16) .., columns 4 (t % 16) ..) of a 64 x 64 matrix in
+// registers (identity beyond M); per pivot k the owners publish row k and
+// column k to LDS (double-buffered: one barrier per pivot) and every thread
+// updates its block.  Without pivoting on a Hermitian positive-definite R
+// the k-th pivot is the Cholesky pivot d_k = L_kk^2, so the reference's
+// factorisation test (d_k > 1e-12, :398-421) is applied to it; a failed
+// channel falls back to the Bartlett form on the folded R (:354-375).
+// Q is written in full (FP64) with its trace; the scan evaluates
+// 1 / (tr Q + 2 Re sum_{r<c} conj(a_r) Q_rc a_c) = 1 / (a^H R^-1 a) in
+// FP64, where the large entries of R^-1 cancel.
+__global__ __launch_bounds__(kThreads) void opt_factor_kernel(
+    const double* __restrict__ state, int M, double2* __restrict__ scan64,
+    double* __restrict__ trace64, float2* __restrict__ scanmat, float* __restrict__ trace_out,
+    int* __restrict__ mode_out) {
+    __shared__ cd rowb[2][64];
+    __shared__ cd colb[2][64];
+    __shared__ double s_diag[64];
+    const int f = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int i0 = 4 * (tid >> 4), j0 = 4 * (tid & 15);
+    const int MM = M * M;
+    const double* R = state + (size_t)f * MM * 2;
+    cd a[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = i0 + u, j = j0 + v;
+            if (i < M && j < M)
+                a[u][v] = cd{R[2 * (i * M + j)], R[2 * (i * M + j) + 1]};
+            else
+                a[u][v] = cd{i == j ? 1.0 : 0.0, 0.0};
+        }
+    bool ok = true;
+    for (int k = 0; k < 64; ++k) {
+        const int buf = k & 1;
+        // static register indices only (a runtime index would spill a[][])
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u == k) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) rowb[buf][j0 + v] = a[u][v];
+            }
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            if (j0 + v == k) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) colb[buf][i0 + u] = a[u][v];
+            }
+        __syncthreads();
+        const double piv = rowb[buf][k].re;
+        if (k < M && !(piv > 1e-12)) {  // uniform: every thread reads the same pivot
+            ok = false;
+            break;
+        }
+        const double p = 1.0 / piv;
+        cd rj[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) rj[v] = rowb[buf][j0 + v];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u;
+            const cd ci = colb[buf][i];
+            const cd cp{ci.re * p, ci.im * p};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = j0 + v;
+                if (i != k && j != k) {
+                    a[u][v].re -= cp.re * rj[v].re - cp.im * rj[v].im;
+                    a[u][v].im -= cp.re * rj[v].im + cp.im * rj[v].re;
+                } else if (i == k && j != k) {
+                    a[u][v] = cd{rj[v].re * p, rj[v].im * p};
+                } else if (j == k && i != k) {
+                    a[u][v] = cd{-cp.re, -cp.im};
+                } else {
+                    a[u][v] = cd{p, 0.0};
+                }
+            }
+        }
+    }
+    if (!ok) {
+        float2* Q = scanmat + (size_t)f * MM;
+        for (int e = tid; e < MM; e += kThreads) {
+            const int r = e / M, c = e - (e / M) * M;
+            Q[e] = r < c ? make_float2((float)R[2 * e], (float)R[2 * e + 1])
+                         : make_float2(0.0f, 0.0f);
+        }
+        if (tid == 0) {
+            double tr = 0.0;
+            for (int i = 0; i < M; ++i) tr += R[2 * (i * M + i)];
+            trace_out[f] = (float)tr;
+            mode_out[f] = 0;
+        }
+        return;
+    }
+    double2* Q = scan64 + (size_t)f * MM;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = i0 + u, j = j0 + v;
+            if (i < M && j < M) {
+                Q[i * M + j] = make_double2(a[u][v].re, a[u][v].im);
+                if (i == j) s_diag[i] = a[u][v].re;
+            }
+        }
+    __syncthreads();
+    if (tid == 0) {
+        double tr = 0.0;
+        for (int i = 0; i < M; ++i) tr += s_diag[i];
+        trace64[f] = tr;
         mode_out[f] = 1;
     }
 }
@@ -318,14 +354,20 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// One pass of a wave over kNc cells: acc_lane = sum_r op(a_r, Q[r][lane]),
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One pass of a wave over NC cells: acc_lane = sum_r op(a_r, Q[r][lane]),
 // the a_r broadcast from lane r.  BART: conj(a_r) R[r][lane] over the strict
 // upper triangle; else W[lane][r] a_r (Q holds W transposed).  Q is a
 // zero-padded 64 x 64 tile and a_r = 0 for r >= M, so rows are taken four
 // at a time (the four LDS reads issue together) up to M rounded up.
-template <bool BART>
+template <int NC, bool BART>
 __device__ __forceinline__ void scan_pass(const float2* __restrict__ Q, int M, int lane,
-                                          const float2 (&a)[kNc], float2 (&acc)[kNc]) {
+                                          const float2 (&a)[NC], float2 (&acc)[NC]) {
     for (int r0 = 0; r0 < M; r0 += 4) {
         float2 q[4];
 #pragma unroll
@@ -334,7 +376,7 @@ __device__ __forceinline__ void scan_pass(const float2* __restrict__ Q, int M, i
         for (int u = 0; u < 4; ++u) {
             const int r = r0 + u;
 #pragma unroll
-            for (int j = 0; j < kNc; ++j) {
+            for (int j = 0; j < NC; ++j) {
                 const float ar = __builtin_bit_cast(
                     float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a[j].x), r));
                 const float ai = __builtin_bit_cast(
@@ -351,16 +393,16 @@ __device__ __forceinline__ void scan_pass(const float2* __restrict__ Q, int M, i
     }
 }
 
-template <bool BART>
+template <int NC, bool BART>
 __device__ void scan_cells(const float2* __restrict__ Q, const OptSearch& s,
                            const OptGridArgs& g, int level, int c_begin, int c_end, int lane,
                            int wave, int f, int F, double kw, float tr, float px, float py,
                            float pz, float* __restrict__ part) {
     const bool live = lane < g.M;
-    for (int c0 = c_begin + wave * kNc; c0 < c_end; c0 += 4 * kNc) {
-        float2 a[kNc], acc[kNc];
+    for (int c0 = c_begin + wave * NC; c0 < c_end; c0 += 4 * NC) {
+        float2 a[NC], acc[NC];
 #pragma unroll
-        for (int j = 0; j < kNc; ++j) {
+        for (int j = 0; j < NC; ++j) {
             a[j] = make_float2(0.0f, 0.0f);
             acc[j] = make_float2(0.0f, 0.0f);
             const int cell = min(c0 + j, c_end - 1);
@@ -374,9 +416,9 @@ __device__ void scan_cells(const float2* __restrict__ Q, const OptSearch& s,
                 a[j] = make_float2((float)cs, -(float)sn);
             }
         }
-        scan_pass<BART>(Q, g.M, lane, a, acc);
+        scan_pass<NC, BART>(Q, g.M, lane, a, acc);
 #pragma unroll
-        for (int j = 0; j < kNc; ++j) {
+        for (int j = 0; j < NC; ++j) {
             float t;
             if (BART)
                 t = acc[j].x * a[j].x - acc[j].y * a[j].y;  // Re(y_c a_c)
@@ -394,21 +436,93 @@ __device__ void scan_cells(const float2* __restrict__ Q, const OptSearch& s,
     }
 }
 
+__device__ __forceinline__ double readlane_f64(double v, int r) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)b, r);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), r);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// Capon in FP64: P = 1 / (tr Q + 2 Re sum_{r<c} conj(a_r) Q_rc a_c) with
+// Q = R^-1 (strict upper triangle in Q64, zero-padded to 64 x 64).
+template <int NC>
+__device__ void scan_cells_capon64(const double2* __restrict__ Q64, const OptSearch& s,
+                                   const OptGridArgs& g, int level, int c_begin, int c_end,
+                                   int lane, int wave, int f, int F, double kw, double tr,
+                                   float px, float py, float pz, float* __restrict__ part) {
+    const bool live = lane < g.M;
+    for (int c0 = c_begin + wave * NC; c0 < c_end; c0 += 4 * NC) {
+        double2 a[NC], acc[NC];
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            a[j] = make_double2(0.0, 0.0);
+            acc[j] = make_double2(0.0, 0.0);
+            const int cell = min(c0 + j, c_end - 1);
+            float dir[3];
+            double st[3];
+            cell_direction(g, s, level, cell, dir, st);
+            if (live) {
+                const double ph = kw * ((double)px * st[0] + (double)py * st[1] + (double)pz * st[2]);
+                double sn, cs;
+                sincos(ph, &sn, &cs);
+                a[j] = make_double2(cs, -sn);
+            }
+        }
+        for (int r0 = 0; r0 < g.M; r0 += 4) {
+            double2 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = Q64[(r0 + u) * 64 + lane];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int r = r0 + u;
+#pragma unroll
+                for (int j = 0; j < NC; ++j) {
+                    const double ar = readlane_f64(a[j].x, r);
+                    const double ai = readlane_f64(a[j].y, r);
+                    acc[j].x = fma(ai, q[u].y, fma(ar, q[u].x, acc[j].x));
+                    acc[j].y = fma(-ai, q[u].x, fma(ar, q[u].y, acc[j].y));
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            double t = acc[j].x * a[j].x - acc[j].y * a[j].y;
+            t = wave_sum_f64(t);
+            const double den = tr + 2.0 * t;
+            float p = den > 1.0e-12 ? (float)(1.0 / den) : 0.0f;
+            if (!isfinite(p)) p = 0.0f;
+            if (lane == 0 && c0 + j < c_end) part[(size_t)(c0 + j) * F + f] = p;
+        }
+    }
+}
+
 // grid (F, chunks); part[cell][f] = the channel's power at the cell.
+template <int NC, int NC64>
 __global__ __launch_bounds__(kThreads) void opt_scan_kernel(
     const float2* __restrict__ scanmat, const float* __restrict__ trace,
+    const double2* __restrict__ scan64, const double* __restrict__ trace64,
     const int* __restrict__ mode, const float* __restrict__ pos, const float* __restrict__ freqs,
     const OptSearch* __restrict__ search, OptGridArgs g, int level, int n_cells,
     int cells_per_chunk, int F, float* __restrict__ part) {
-    __shared__ __attribute__((aligned(16))) float2 Q[64 * 64];
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    float2* Q = reinterpret_cast<float2*>(smem);
+    double2* Q64 = reinterpret_cast<double2*>(smem);
     const int f = blockIdx.x;
+    const bool capon64 = mode[f] == 1;
     const int M = g.M;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int c_begin = (int)blockIdx.y * cells_per_chunk;
     const int c_end = min(n_cells, c_begin + cells_per_chunk);
-    {
+    if (capon64) {
+        // strict upper triangle of R^-1, zero elsewhere
+        const double2* src = scan64 + (size_t)f * M * M;
+        for (int e = tid; e < 64 * 64; e += kThreads) {
+            const int r = e >> 6, c = e & 63;
+            Q64[e] = (r < c && c < M) ? src[r * M + c] : make_double2(0.0, 0.0);
+        }
+    } else {
         // [64][64], zero outside [M][M] (M may be odd: element by element)
         const float2* src = scanmat + (size_t)f * M * M;
         for (int e = tid; e < 64 * 64; e += kThreads) {
@@ -426,21 +540,15 @@ __global__ __launch_bounds__(kThreads) void opt_scan_kernel(
         py = pos[3 * lane + 1];
         pz = pos[3 * lane + 2];
     }
-    if (mode[f] == 0)
-        scan_cells<true>(Q, s, g, level, c_begin, c_end, lane, wave, f, F, kw, tr, px, py, pz,
-                         part);
+    if (capon64)
+        scan_cells_capon64<NC64>(Q64, s, g, level, c_begin, c_end, lane, wave, f, F, kw, trace64[f],
+                           px, py, pz, part);
     else
-        scan_cells<false>(Q, s, g, level, c_begin, c_end, lane, wave, f, F, kw, tr, px, py, pz,
-                          part);
+        scan_cells<NC, true>(Q, s, g, level, c_begin, c_end, lane, wave, f, F, kw, tr, px, py, pz,
+                         part);
 }
 
 // ---------------------------------------------------------------- decide
-__device__ __forceinline__ double wave_sum_f64(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 constexpr int kDecideThreads = 1024;
 
 // One workgroup: each wave sums a cell's channel powers (FP64, a fixed
@@ -453,12 +561,24 @@ __global__ __launch_bounds__(kDecideThreads) void opt_decide_kernel(
     float* P = reinterpret_cast<float*>(smem);
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    for (int cell = wave; cell < n_cells; cell += kDecideThreads / 64) {
-        const float* pc = part + (size_t)cell * F;
-        double sum = 0.0;
-        for (int f = lane; f < F; f += 64) sum += (double)pc[f];
-        sum = wave_sum_f64(sum);
-        if (lane == 0) P[cell] = (float)sum;
+    // four cells per wave at a time, so that their loads are in flight
+    // together
+    constexpr int kWaves = kDecideThreads / 64;
+    for (int c0 = wave; c0 < n_cells; c0 += 4 * kWaves) {
+        double sum[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int f = lane; f < F; f += 64) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int cell = c0 + j * kWaves;
+                if (cell < n_cells) sum[j] += (double)part[(size_t)cell * F + f];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double v = wave_sum_f64(sum[j]);
+            const int cell = c0 + j * kWaves;
+            if (lane == 0 && cell < n_cells) P[cell] = (float)v;
+        }
     }
     __syncthreads();
     if (tid != 0) return;
@@ -511,31 +631,45 @@ __global__ __launch_bounds__(kDecideThreads) void opt_decide_kernel(
 
 void launch_opt_fold(hipStream_t stream, const int* rs, double* state, bool first, size_t F,
                      size_t M, size_t K, size_t L, bool fb, bool capon, float load,
-                     float lambda, float2* scanmat, float* trace, int* mode) {
-    const size_t smem = 2 * M * M * sizeof(cd);
-    hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(opt_fold_kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
-              "hipFuncSetAttribute");
-    hipLaunchKernelGGL(opt_fold_kernel, dim3((unsigned)F), dim3(kThreads), smem, stream, rs,
-                       state, first ? 1 : 0, (int)M, (int)K, (int)L, fb ? 1 : 0, capon ? 1 : 0,
-                       load, lambda, scanmat, trace, mode);
+                     float lambda, float2* scanmat, float* trace, double2* scan64,
+                     double* trace64, int* mode) {
+    hipLaunchKernelGGL(opt_fold_kernel, dim3((unsigned)F), dim3(kThreads), 0, stream, rs, state,
+                       first ? 1 : 0, (int)M, (int)K, (int)L, fb ? 1 : 0, capon ? 1 : 0, load,
+                       lambda, scanmat, trace, mode);
     hip_check(hipGetLastError(), "opt_fold_kernel launch");
+    if (!capon) return;
+    hipLaunchKernelGGL(opt_factor_kernel, dim3((unsigned)F), dim3(kThreads), 0, stream, state,
+                       (int)M, scan64, trace64, scanmat, trace, mode);
+    hip_check(hipGetLastError(), "opt_factor_kernel launch");
 }
 
 void launch_opt_scan(hipStream_t stream, const float2* scanmat, const float* trace,
-                     const int* mode, const float* pos, const float* freqs,
-                     const OptSearch* search, const OptGridArgs& g, int level, size_t F,
-                     size_t n_cells, float* part) {
-    // enough workgroups to cover the CUs twice; each stages its channel's
-    // operand once
-    const size_t per_pass = 4 * kNc;
-    size_t chunks = std::max<size_t>(1, ceil_div(512, F));
-    chunks = std::min(chunks, ceil_div(n_cells, per_pass));
-    const size_t cpc = ceil_div(ceil_div(n_cells, chunks), per_pass) * per_pass;
-    chunks = ceil_div(n_cells, cpc);
-    hipLaunchKernelGGL(opt_scan_kernel, dim3((unsigned)F, (unsigned)chunks), dim3(kThreads), 0,
-                       stream, scanmat, trace, mode, pos, freqs, search, g, level, (int)n_cells,
-                       (int)cpc, (int)F, part);
+                     const double2* scan64, const double* trace64, const int* mode,
+                     const float* pos, const float* freqs, const OptSearch* search,
+                     const OptGridArgs& g, int level, size_t F, size_t n_cells, bool capon,
+                     float* part) {
+    // One pass per wave: cells are spread over as many workgroups as it
+    // takes (each stages its channel's operand once), which keeps the
+    // per-wave chain short; the 3 x 3 refinement uses three waves of 3.
+    const bool coarse = level == 0;
+    const size_t per_wave = coarse ? (capon ? 4 : 8) : 3;
+    const size_t cpc = 4 * per_wave;
+    const size_t chunks = ceil_div(n_cells, cpc);
+    const size_t smem = 64 * 64 * (capon ? sizeof(double2) : sizeof(float2));
+    const void* fn = coarse ? reinterpret_cast<const void*>(opt_scan_kernel<8, 4>)
+                            : reinterpret_cast<const void*>(opt_scan_kernel<3, 3>);
+    if (capon)
+        hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
+                  "hipFuncSetAttribute");
+#define BF_OPT_SCAN(NC_, NC64_)                                                                \
+    hipLaunchKernelGGL((opt_scan_kernel<NC_, NC64_>), dim3((unsigned)F, (unsigned)chunks),      \
+                       dim3(kThreads), smem, stream, scanmat, trace, scan64, trace64, mode, pos,  \
+                       freqs, search, g, level, (int)n_cells, (int)cpc, (int)F, part)
+    if (coarse)
+        BF_OPT_SCAN(8, 4);
+    else
+        BF_OPT_SCAN(3, 3);
+#undef BF_OPT_SCAN
     hip_check(hipGetLastError(), "opt_scan_kernel launch");
 }
 
