@@ -440,6 +440,63 @@ bf_status bf_doa_last_timings(bf_doa* doa, bf_doa_timings* timings);
 bf_status bf_doa_get_state(bf_doa* doa, double* state, int* initialised);
 bf_status bf_doa_set_state(bf_doa* doa, const double* state, int initialised);
 
+/* ------------------------------------ geometry, beam grids, weights */
+/* Producers of the beamformer's weight inputs (src/geometry.cpp:33-451,
+ * src/weights.cpp:11-122), same arithmetic and messages.  Positions and
+ * directions are [n][3] float, (u, v) centres [n][2]; weights are
+ * interleaved complex f32. */
+/* regular_array (geometry.cpp:33-51): x = column * spacing, y = row * spacing. */
+bf_status bf_regular_array(size_t rows, size_t columns, float spacing_m, float* out);
+/* default_positions (geometry.cpp:53-67): 4x8, 8x8, 8x16, 16x16 grids. */
+bf_status bf_default_positions(size_t n_ant, float spacing_m, float* out);
+/* constant_frequencies / channelized_frequencies (geometry.cpp:69-95). */
+bf_status bf_constant_frequencies(size_t n_freq, float frequency_hz, float* out);
+bf_status bf_channelized_frequencies(size_t n_freq, float start_hz, float channel_width_hz,
+                                     float* out);
+/* direction_from_lm (geometry.cpp:97-106). */
+bf_status bf_direction_from_lm(float l, float m, float* out);
+/* default_beam_grid, the line of n_beams beams at l_step (geometry.cpp:108-125). */
+bf_status bf_default_beam_grid(size_t n_beams, float l_step, float m, float* out);
+/* centered_integer_range (geometry.cpp:127-137). */
+bf_status bf_centered_integer_range(size_t count, long long* out);
+/* select_fft_beam_centers_rectangular (geometry.cpp:139-215): centers
+ * [n_beams][2]; shape = {n_u, n_v, n_bank_u, n_bank_v}. */
+bf_status bf_select_fft_beam_centers_rectangular(float du_fft_u, float dv_fft_v,
+                                                 size_t n_beams, size_t m_side, size_t n_side,
+                                                 float* centers, size_t* shape);
+/* fft_beam_grid, the FFT-bin grid (geometry.cpp:217-249). */
+bf_status bf_fft_beam_grid(size_t n_ant, size_t n_beams, float spacing_m,
+                           float design_frequency_hz, float* out);
+/* estimate_nbeams_hex_formula (geometry.cpp:251-273). */
+bf_status bf_estimate_nbeams_hex_formula(float spacing_m, size_t m_side, size_t n_side,
+                                         float bw_e_deg, float bw_h_deg, float wavelength_m,
+                                         size_t* out);
+/* generate_hex_targets_cropped_fov, the hexagonal grid (geometry.cpp:275-330):
+ * up to n_beams (u, v) targets in *n_out; out may be NULL when n_beams = 0. */
+bf_status bf_generate_hex_targets_cropped_fov(size_t n_beams, float u_max, float v_max,
+                                              float* out, size_t* n_out);
+/* rectangular_beam_grid (geometry.cpp:332-366): n_ant beams. */
+bf_status bf_rectangular_beam_grid(size_t n_ant, float spacing_m, float design_frequency_hz,
+                                   float* out);
+/* load_positions / load_frequencies / load_directions (geometry.cpp:368-451). */
+bf_status bf_load_positions(const char* path, size_t expected_count, float* out);
+bf_status bf_load_frequencies(const char* path, size_t expected_count, float* out);
+bf_status bf_load_directions(const char* path, size_t expected_count, float* out);
+/* generate_weights [B][F][A] / generate_tiled_weights [F][ceil(B/32)][A][32]
+ * on the host (weights.cpp:11-122). */
+bf_status bf_generate_weights(const bf_dims* dims, const float* positions,
+                              const float* frequencies, const float* directions, float* out,
+                              size_t n_complex);
+bf_status bf_generate_tiled_weights(const bf_dims* dims, const float* positions,
+                                    const float* frequencies, const float* directions,
+                                    float* out, size_t n_complex);
+/* The same weights generated on the device into d_weights (canonical, or
+ * tiled when `tiled`), asynchronous on `stream`; inputs are device arrays
+ * and are not validated here (the host forms validate). */
+bf_status bf_generate_weights_device(void* stream, const bf_dims* dims,
+                                     const float* d_positions, const float* d_frequencies,
+                                     const float* d_directions, float* d_weights, int tiled);
+
 /* ---------------------------------------- adaptive planar DOA tracker */
 /* GPU counterpart of the reference's CpuOptBeamTracker
  * (include/beamformer/cpu_opt_beam_tracker.hpp:39-196,

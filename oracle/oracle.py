@@ -347,6 +347,11 @@ def ref_lib():
         "ref_read_packed_shard": [C.c_char_p, C.c_char_p, C.c_char_p, _sz, _sz, _sz, _sz,
                                   _u8p, _u8p, C.POINTER(C.c_uint64)],
         "ref_write_tiled_weights_file": [C.c_char_p, _f32p, _sz, _sz, _sz, _sz],
+        "ref_hex_targets": [_sz, C.c_float, C.c_float, _f32p, C.POINTER(_sz)],
+        "ref_estimate_hex": [C.c_float, _sz, _sz, C.c_float, C.c_float, C.c_float,
+                             C.POINTER(_sz)],
+        "ref_select_fft": [C.c_float, C.c_float, _sz, _sz, _sz, _f32p, C.POINTER(_sz)],
+        "ref_fft_beam_grid_at": [_sz, _sz, C.c_float, C.c_float, _f32p],
         "ref_cpu_opt_tracker": [_u8p, _sz, _sz, _sz, _sz, _f32p, _f32p, _f32p, _f32p, _sz,
                                 C.c_int, _sz, _sz, C.c_float, C.c_float, _sz, C.c_float,
                                 C.c_float, C.c_int, _f32p, _f32p],

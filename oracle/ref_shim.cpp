@@ -83,6 +83,44 @@ int ref_fft_beam_grid(std::size_t n_ant, std::size_t n_beams, float* out) {
     return guarded([&] { put_vec3s(fft_beam_grid(n_ant, n_beams), out); });
 }
 
+int ref_hex_targets(std::size_t n, float u_max, float v_max, float* out, std::size_t* n_out) {
+    return guarded([&] {
+        const auto v = generate_hex_targets_cropped_fov(n, u_max, v_max);
+        *n_out = v.size();
+        for (std::size_t i = 0; i < v.size(); ++i) {
+            out[2 * i] = v[i][0];
+            out[2 * i + 1] = v[i][1];
+        }
+    });
+}
+
+int ref_estimate_hex(float spacing, std::size_t m_side, std::size_t n_side, float bw_e,
+                     float bw_h, float wavelength, std::size_t* out) {
+    return guarded([&] {
+        *out = estimate_nbeams_hex_formula(spacing, m_side, n_side, bw_e, bw_h, wavelength);
+    });
+}
+
+int ref_select_fft(float du, float dv, std::size_t n, std::size_t m_side, std::size_t n_side,
+                   float* centers, std::size_t* shape) {
+    return guarded([&] {
+        const auto s = select_fft_beam_centers_rectangular(du, dv, n, m_side, n_side);
+        for (std::size_t i = 0; i < s.centers.size(); ++i) {
+            centers[2 * i] = s.centers[i][0];
+            centers[2 * i + 1] = s.centers[i][1];
+        }
+        shape[0] = s.n_u;
+        shape[1] = s.n_v;
+        shape[2] = s.n_bank_u;
+        shape[3] = s.n_bank_v;
+    });
+}
+
+int ref_fft_beam_grid_at(std::size_t n_ant, std::size_t n_beams, float spacing, float design,
+                         float* out) {
+    return guarded([&] { put_vec3s(fft_beam_grid(n_ant, n_beams, spacing, design), out); });
+}
+
 int ref_generate_weights(std::size_t B, std::size_t F, std::size_t A, const float* pos,
                          const float* freqs, const float* dirs, float* out) {
     return guarded([&] {

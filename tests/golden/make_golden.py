@@ -15,6 +15,8 @@ Each case mirrors a case of the reference's own tests:
   tracker_moving_a32    tests/cuda/test_cuda_tracker_v2.cpp:45-58 (shortened)
   io_quantized_meta.txt src/io.cpp:149-183 (write_quantized_output_metadata)
   io_shard_*.bin/.txt   src/io.cpp:244-278 (write_packed_shard, T=2 A=32)
+  geometry_grids        tests/cpu/test_geometry.cpp (rectangular, FFT-bin, hex and
+                        line grids, FFT bin selection, hex beam estimates)
   opt_tracker_*         tests/cpu/test_cpu_opt_beam_tracker.cpp:290-645 (the
                         adaptive tracker: Bartlett, Capon, coarse-only with
                         interpolation, A=64, forgetting on a moving source,
@@ -174,6 +176,48 @@ def main() -> None:
                                     os.path.join(HERE, "io_shard_meta.txt").encode(),
                                     os.path.join(HERE, "io_shard_mask.bin").encode(),
                                     payload, mask, T, F, A, 5, 0, 1234567890123, 2560) == 0
+    # Beam-grid generators (src/geometry.cpp:108-366), cases of
+    # tests/cpu/test_geometry.cpp plus off-default spacings.
+    import ctypes as C
+    grids = {}
+    for A in (32, 64):
+        r = np.zeros((A, 3), np.float32)
+        assert R.ref_rectangular_beam_grid(A, 0.6, 400e6, r) == 0
+        grids[f"rect_a{A}"] = r
+        r2 = np.zeros((A, 3), np.float32)
+        assert R.ref_rectangular_beam_grid(A, 0.7, 450e6, r2) == 0
+        grids[f"rect_a{A}_s07_f450"] = r2
+        for B in (1, 7, 128):
+            g = np.zeros((B, 3), np.float32)
+            assert R.ref_fft_beam_grid(A, B, g) == 0
+            grids[f"fft_a{A}_b{B}"] = g
+        g = np.zeros((50, 3), np.float32)
+        assert R.ref_fft_beam_grid_at(A, 50, 0.5, 450e6, g) == 0
+        grids[f"fft_a{A}_b50_s05_f450"] = g
+    for n, um, vm in ((128, 0.8, 0.6), (37, 0.3, 0.5), (1, 0.2, 0.2)):
+        h = np.zeros((n, 2), np.float32)
+        cnt = C.c_size_t()
+        assert R.ref_hex_targets(n, um, vm, h, C.byref(cnt)) == 0
+        grids[f"hex_{n}_{um}_{vm}"] = h[:cnt.value]
+    est = []
+    for args in ((0.6, 8, 8, 108.0, 74.0, 0.75), (0.6, 8, 4, 60.0, 40.0, 0.9993),
+                 (0.3, 16, 16, 20.0, 30.0, 1.5)):
+        e = C.c_size_t()
+        assert R.ref_estimate_hex(*args, C.byref(e)) == 0
+        est.append(e.value)
+    grids["hex_estimates"] = np.array(est, np.int64)
+    for B, m_side, n_side in ((128, 8, 4), (128, 8, 8), (13, 8, 8)):
+        cen = np.zeros((B, 2), np.float32)
+        shape = (C.c_size_t * 4)()
+        du = np.float32(0.0390303)
+        assert R.ref_select_fft(du, du * np.float32(2.0 if n_side == 4 else 1.0), B, m_side,
+                                n_side, cen, shape) == 0
+        grids[f"select_{B}_{m_side}x{n_side}"] = cen
+        grids[f"select_{B}_{m_side}x{n_side}_shape"] = np.array(list(shape), np.int64)
+    d7 = np.zeros((7, 3), np.float32)
+    assert R.ref_default_beam_grid(7, 0.015, 0.05, d7) == 0
+    grids["line_7_0.015_0.05"] = d7
+    save("geometry_grids", **grids)
     # Adaptive DOA tracker (CpuOptBeamTracker::run_into) cases.
     for name, case in OPT_CASES.items():
         A, T, n = case["A"], case["T"], case.get("calls", 1)

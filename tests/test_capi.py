@@ -7,7 +7,8 @@ import re
 
 import pytest
 
-from beamform_project_amd import _capi, api, doa, opt_tracker, serve, tracker  # noqa: F401  (register the ABI)
+from beamform_project_amd import (_capi, api, doa, formats, geometry,  # noqa: F401
+                                  opt_tracker, serve, tracker)  # (register the ABI)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
