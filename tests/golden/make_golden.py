@@ -68,6 +68,11 @@ OPT_CASES = {
     "scan_off": dict(A=32, T=8, S=2, source=(0.02, -0.01, 1e-3, 5e-4),
                      start=(0.02, -0.01, 0.99975), rate=(1e-3, 5e-4), estimator=0, G=1,
                      levels=0, fov=0.2, p_sub=0, lam=1.0),
+    "partial_odd_capon": dict(A=64, T=7, S=3, source=(-0.02, 0.03, 2e-3, -1e-3),
+                              start=(0.0, 0.0, 1.0), rate=(0.0, 0.0), estimator=1, G=5,
+                              levels=3, fov=0.15, p_sub=0, lam=0.8),
+    "edge_peak": dict(A=32, T=4, S=4, source=(0.17, -0.12, 0.0, 0.0), start=_Z,
+                      rate=(0.0, 0.0), estimator=0, G=8, levels=0, fov=0.1, p_sub=0, lam=1.0),
     "smoothing_bartlett": dict(A=32, T=4, S=4, source=(0.05, -0.03, 0.0, 0.0), start=_Z,
                                rate=(0.0, 0.0), estimator=0, G=16, levels=1, fov=0.2, p_sub=24,
                                lam=1.0),

@@ -4,7 +4,9 @@ CpuOptBeamTracker (src/cpu_opt_beam_tracker.cpp:1619-2457).
 Fixtures opt_tracker_* hold the reference's window directions and emitted
 intensities for the cases of tests/cpu/test_cpu_opt_beam_tracker.cpp
 (Bartlett, Capon, coarse grid with interpolation, A=64, forgetting on a
-moving source, state carried across calls, scan disabled).  Bars: every
+moving source, state carried across calls, scan disabled, a partial last
+window on an odd grid with three refinement levels, a peak on the grid
+edge).  Bars: every
 window's search decision equals the reference's, so the directions agree to
 float rounding (1e-6), and the intensities meet the reference tracker
 tolerance (1e-4 relative, 1e-5 absolute) for every emission kernel.
@@ -24,7 +26,7 @@ from helpers import F, golden, tracker_close
 pytestmark = pytest.mark.gpu
 K = api.Kernel
 CASES = ["bartlett_static", "capon_static", "coarse_interp", "a64_bartlett",
-         "moving_forgetting", "capon_stateful", "scan_off"]
+         "moving_forgetting", "capon_stateful", "scan_off", "partial_odd_capon", "edge_peak"]
 
 
 def case_input(c):
