@@ -82,13 +82,20 @@ struct bf_opt_tracker {
     hipStream_t last_stream = nullptr;
     float* d_pos = nullptr;
     float* d_freqs = nullptr;
-    int* d_rs = nullptr;
     double* d_state = nullptr;
-    float2* d_scan = nullptr;
-    float* d_trace = nullptr;
-    double2* d_scan64 = nullptr;  // Capon: R^-1 per channel
-    double* d_trace64 = nullptr;
-    int* d_mode = nullptr;
+    // Per-window operands, two slots: the covariance, fold and factor of
+    // window w + 1 run on the prep stream while window w is searched.
+    struct Slot {
+        int* rs = nullptr;
+        float2* scan = nullptr;
+        float* trace = nullptr;
+        double2* scan64 = nullptr;  // Capon: R^-1 per channel
+        double* trace64 = nullptr;
+        int* mode = nullptr;
+    } slot[2];
+    hipStream_t prep = nullptr;
+    hipEvent_t ev_start = nullptr, ev_ready[2] = {nullptr, nullptr},
+               ev_used[2] = {nullptr, nullptr};
     float* d_part = nullptr;
     OptSearch* d_search = nullptr;
     float* d_dirs = nullptr;
@@ -97,10 +104,17 @@ struct bf_opt_tracker {
 
     ~bf_opt_tracker() {
         if (last_stream) (void)hipStreamSynchronize(last_stream);
-        for (void* p : {(void*)d_pos, (void*)d_freqs, (void*)d_rs, (void*)d_state, (void*)d_scan,
-                        (void*)d_trace, (void*)d_scan64, (void*)d_trace64, (void*)d_mode,
-                        (void*)d_part, (void*)d_search, (void*)d_dirs, (void*)d_traj})
+        if (prep) (void)hipStreamSynchronize(prep);
+        for (const Slot& q : slot)
+            for (void* p : {(void*)q.rs, (void*)q.scan, (void*)q.trace, (void*)q.scan64,
+                            (void*)q.trace64, (void*)q.mode})
+                if (p) (void)hipFree(p);
+        for (void* p : {(void*)d_pos, (void*)d_freqs, (void*)d_state, (void*)d_part,
+                        (void*)d_search, (void*)d_dirs, (void*)d_traj})
             if (p) (void)hipFree(p);
+        for (hipEvent_t e : {ev_start, ev_ready[0], ev_ready[1], ev_used[0], ev_used[1]})
+            if (e) (void)hipEventDestroy(e);
+        if (prep) (void)hipStreamDestroy(prep);
     }
 
     OptGridArgs grid_args() const {
@@ -165,20 +179,32 @@ struct bf_opt_tracker {
                   "hipMemcpyAsync search");
         const OptGridArgs g = grid_args();
         const int last = (int)cfg.refinement_levels;
+        // The prep stream starts after the caller's prior work (the input).
+        hip_check(hipEventRecord(ev_start, stream), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(prep, ev_start, 0), "hipStreamWaitEvent");
         for (size_t w = 0; w < W; ++w) {
+            const int k = (int)(w & 1);
+            Slot& q = slot[k];
+            // prep: window w's operands into slot k, once window w - 2's
+            // search has released it
+            if (w >= 2) hip_check(hipStreamWaitEvent(prep, ev_used[k], 0), "hipStreamWaitEvent");
             const size_t t0 = w * S;
             const size_t K = std::min(S, T - t0);
-            launch_doa_covariance(stream, x, T, F, A, t0, K, M, d_rs, nullptr);
-            launch_opt_fold(stream, d_rs, d_state, !initialised, F, M, K, L, smoothing, capon,
-                            cfg.diagonal_load, cfg.forgetting_factor, d_scan, d_trace,
-                            d_scan64, d_trace64, d_mode);
+            launch_doa_covariance(prep, x, T, F, A, t0, K, M, q.rs, nullptr);
+            launch_opt_fold(prep, q.rs, d_state, !initialised, F, M, K, L, smoothing, capon,
+                            cfg.diagonal_load, cfg.forgetting_factor, q.scan, q.trace, q.scan64,
+                            q.trace64, q.mode);
             initialised = true;
+            hip_check(hipEventRecord(ev_ready[k], prep), "hipEventRecord");
+            // search of window w on the caller's stream
+            hip_check(hipStreamWaitEvent(stream, ev_ready[k], 0), "hipStreamWaitEvent");
             for (int level = 0; level <= last; ++level) {
                 const size_t cells = level == 0 ? G * G : 9;
-                launch_opt_scan(stream, d_scan, d_trace, d_scan64, d_trace64, d_mode, d_pos,
+                launch_opt_scan(stream, q.scan, q.trace, q.scan64, q.trace64, q.mode, d_pos,
                                 d_freqs, d_search, g, level, F, cells, capon, d_part);
                 launch_opt_decide(stream, d_part, F, cells, g, level, last, w, d_search, d_dirs);
             }
+            hip_check(hipEventRecord(ev_used[k], stream), "hipEventRecord");
         }
         L_emit.dir_source = kDirTable;
         L_emit.dir_table = d_dirs;
@@ -266,15 +292,22 @@ void create_tracker(const float* positions, const float* frequencies, const bf_d
               "hipMemcpy positions");
     hip_check(hipMemcpy(t->d_freqs, fr.data(), F * sizeof(float), hipMemcpyHostToDevice),
               "hipMemcpy frequencies");
-    t->d_rs = dmalloc<int>(F * M * M * 2);
     t->d_state = dmalloc<double>(F * M * M * 2);
-    t->d_scan = dmalloc<float2>(F * M * M);
-    t->d_trace = dmalloc<float>(F);
-    if (t->capon) {
-        t->d_scan64 = dmalloc<double2>(F * M * M);
-        t->d_trace64 = dmalloc<double>(F);
+    for (auto& q : t->slot) {
+        q.rs = dmalloc<int>(F * M * M * 2);
+        q.scan = dmalloc<float2>(F * M * M);
+        q.trace = dmalloc<float>(F);
+        if (t->capon) {
+            q.scan64 = dmalloc<double2>(F * M * M);
+            q.trace64 = dmalloc<double>(F);
+        }
+        q.mode = dmalloc<int>(F);
     }
-    t->d_mode = dmalloc<int>(F);
+    // default priority (a stream of another priority is never created here)
+    hip_check(hipStreamCreateWithFlags(&t->prep, hipStreamNonBlocking), "hipStreamCreate");
+    for (hipEvent_t* e : {&t->ev_start, &t->ev_ready[0], &t->ev_ready[1], &t->ev_used[0],
+                          &t->ev_used[1]})
+        hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
     t->d_part = dmalloc<float>(F * std::max<size_t>(t->G * t->G, 9));
     t->d_search = dmalloc<OptSearch>(1);
     t->d_traj = dmalloc<float>(4);

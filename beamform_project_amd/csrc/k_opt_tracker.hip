@@ -366,8 +366,9 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 // zero-padded 64 x 64 tile and a_r = 0 for r >= M, so rows are taken four
 // at a time (the four LDS reads issue together) up to M rounded up.
 template <int NC, bool BART>
-__device__ __forceinline__ void scan_pass(const float2* __restrict__ Q, int M, int lane,
-                                          const float2 (&a)[NC], float2 (&acc)[NC]) {
+__device__ __forceinline__ void scan_pass(const float2* __restrict__ Q,
+                                          const float2* __restrict__ sa, int M, int lane,
+                                          float2 (&acc)[NC]) {
     for (int r0 = 0; r0 < M; r0 += 4) {
         float2 q[4];
 #pragma unroll
@@ -377,10 +378,8 @@ __device__ __forceinline__ void scan_pass(const float2* __restrict__ Q, int M, i
             const int r = r0 + u;
 #pragma unroll
             for (int j = 0; j < NC; ++j) {
-                const float ar = __builtin_bit_cast(
-                    float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a[j].x), r));
-                const float ai = __builtin_bit_cast(
-                    float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a[j].y), r));
+                const float2 av = sa[j * 64 + r];  // broadcast
+                const float ar = av.x, ai = av.y;
                 if (BART) {
                     acc[j].x = fmaf(ai, q[u].y, fmaf(ar, q[u].x, acc[j].x));
                     acc[j].y = fmaf(-ai, q[u].x, fmaf(ar, q[u].y, acc[j].y));
@@ -394,7 +393,8 @@ __device__ __forceinline__ void scan_pass(const float2* __restrict__ Q, int M, i
 }
 
 template <int NC, bool BART>
-__device__ void scan_cells(const float2* __restrict__ Q, const OptSearch& s,
+__device__ void scan_cells(const float2* __restrict__ Q, float2* __restrict__ sa,
+                           const OptSearch& s,
                            const OptGridArgs& g, int level, int c_begin, int c_end, int lane,
                            int wave, int f, int F, double kw, float tr, float px, float py,
                            float pz, float* __restrict__ part) {
@@ -415,8 +415,11 @@ __device__ void scan_cells(const float2* __restrict__ Q, const OptSearch& s,
                 sincos(ph, &sn, &cs);
                 a[j] = make_float2((float)cs, -(float)sn);
             }
+            sa[j * 64 + lane] = a[j];
         }
-        scan_pass<NC, BART>(Q, g.M, lane, a, acc);
+        // the wave's own LDS writes precede its reads in order
+        __builtin_amdgcn_wave_barrier();
+        scan_pass<NC, BART>(Q, sa, g.M, lane, acc);
 #pragma unroll
         for (int j = 0; j < NC; ++j) {
             float t;
@@ -436,17 +439,11 @@ __device__ void scan_cells(const float2* __restrict__ Q, const OptSearch& s,
     }
 }
 
-__device__ __forceinline__ double readlane_f64(double v, int r) {
-    const long long b = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_readlane((int)b, r);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), r);
-    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
-}
-
 // Capon in FP64: P = 1 / (tr Q + 2 Re sum_{r<c} conj(a_r) Q_rc a_c) with
 // Q = R^-1 (strict upper triangle in Q64, zero-padded to 64 x 64).
 template <int NC>
-__device__ void scan_cells_capon64(const double2* __restrict__ Q64, const OptSearch& s,
+__device__ void scan_cells_capon64(const double2* __restrict__ Q64, double2* __restrict__ sa,
+                                   const OptSearch& s,
                                    const OptGridArgs& g, int level, int c_begin, int c_end,
                                    int lane, int wave, int f, int F, double kw, double tr,
                                    float px, float py, float pz, float* __restrict__ part) {
@@ -467,7 +464,9 @@ __device__ void scan_cells_capon64(const double2* __restrict__ Q64, const OptSea
                 sincos(ph, &sn, &cs);
                 a[j] = make_double2(cs, -sn);
             }
+            sa[j * 64 + lane] = a[j];
         }
+        __builtin_amdgcn_wave_barrier();
         for (int r0 = 0; r0 < g.M; r0 += 4) {
             double2 q[4];
 #pragma unroll
@@ -477,8 +476,8 @@ __device__ void scan_cells_capon64(const double2* __restrict__ Q64, const OptSea
                 const int r = r0 + u;
 #pragma unroll
                 for (int j = 0; j < NC; ++j) {
-                    const double ar = readlane_f64(a[j].x, r);
-                    const double ai = readlane_f64(a[j].y, r);
+                    const double2 av = sa[j * 64 + r];  // broadcast
+                    const double ar = av.x, ai = av.y;
                     acc[j].x = fma(ai, q[u].y, fma(ar, q[u].x, acc[j].x));
                     acc[j].y = fma(-ai, q[u].x, fma(ar, q[u].y, acc[j].y));
                 }
@@ -503,12 +502,14 @@ __global__ __launch_bounds__(kThreads) void opt_scan_kernel(
     const double2* __restrict__ scan64, const double* __restrict__ trace64,
     const int* __restrict__ mode, const float* __restrict__ pos, const float* __restrict__ freqs,
     const OptSearch* __restrict__ search, OptGridArgs g, int level, int n_cells,
-    int cells_per_chunk, int F, float* __restrict__ part) {
+    int cells_per_chunk, int F, int capon_run, float* __restrict__ part) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     float2* Q = reinterpret_cast<float2*>(smem);
     double2* Q64 = reinterpret_cast<double2*>(smem);
     const int f = blockIdx.x;
     const bool capon64 = mode[f] == 1;
+    // per-wave steering vectors [NC][64] after the operand tile
+    unsigned char* steer = smem + 64 * 64 * (capon_run ? sizeof(double2) : sizeof(float2));
     const int M = g.M;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -541,10 +542,12 @@ __global__ __launch_bounds__(kThreads) void opt_scan_kernel(
         pz = pos[3 * lane + 2];
     }
     if (capon64)
-        scan_cells_capon64<NC64>(Q64, s, g, level, c_begin, c_end, lane, wave, f, F, kw, trace64[f],
+        scan_cells_capon64<NC64>(Q64, reinterpret_cast<double2*>(steer) + wave * NC64 * 64, s,
+                                 g, level, c_begin, c_end, lane, wave, f, F, kw, trace64[f],
                            px, py, pz, part);
     else
-        scan_cells<NC, true>(Q, s, g, level, c_begin, c_end, lane, wave, f, F, kw, tr, px, py, pz,
+        scan_cells<NC, true>(Q, reinterpret_cast<float2*>(steer) + wave * NC * 64, s, g,
+                             level, c_begin, c_end, lane, wave, f, F, kw, tr, px, py, pz,
                          part);
 }
 
@@ -655,16 +658,21 @@ void launch_opt_scan(hipStream_t stream, const float2* scanmat, const float* tra
     const size_t per_wave = coarse ? (capon ? 4 : 8) : 3;
     const size_t cpc = 4 * per_wave;
     const size_t chunks = ceil_div(n_cells, cpc);
-    const size_t smem = 64 * 64 * (capon ? sizeof(double2) : sizeof(float2));
+    // operand tile, then per-wave steering vectors (sized for either path)
+    const size_t nc = coarse ? 8 : 3, nc64 = coarse ? 4 : 3;  // the template arguments
+    const size_t steer_per_wave =
+        64 * std::max(nc * sizeof(float2), capon ? nc64 * sizeof(double2) : 0);
+    const size_t smem = 64 * 64 * (capon ? sizeof(double2) : sizeof(float2)) + 4 * steer_per_wave;
     const void* fn = coarse ? reinterpret_cast<const void*>(opt_scan_kernel<8, 4>)
                             : reinterpret_cast<const void*>(opt_scan_kernel<3, 3>);
-    if (capon)
+    if (smem > 64 * 1024)
         hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
                   "hipFuncSetAttribute");
 #define BF_OPT_SCAN(NC_, NC64_)                                                                \
     hipLaunchKernelGGL((opt_scan_kernel<NC_, NC64_>), dim3((unsigned)F, (unsigned)chunks),      \
                        dim3(kThreads), smem, stream, scanmat, trace, scan64, trace64, mode, pos,  \
-                       freqs, search, g, level, (int)n_cells, (int)cpc, (int)F, part)
+                       freqs, search, g, level, (int)n_cells, (int)cpc, (int)F, capon ? 1 : 0,  \
+                       part)
     if (coarse)
         BF_OPT_SCAN(8, 4);
     else
