@@ -2,7 +2,9 @@
 // C-ABI's status codes onto the reference's exception types, and its file
 // layer (amd_io.hpp) round-trips the reference's formats.  Runs without a
 // GPU: every device call below fails validation before touching the device.
+#include <cmath>
 #include <cstdio>
+#include <fstream>
 #include <cstdlib>
 #include <unistd.h>
 #include <cstring>
@@ -11,6 +13,7 @@
 
 #include "beamformer/amd.hpp"
 #include "beamformer/amd_io.hpp"
+#include "beamformer/amd_opt_tracker.hpp"
 
 namespace bfa = beamformer::amd;
 
@@ -124,6 +127,62 @@ int main() {
             [&] { bfa::write_quantized_output_metadata(base + "_q2.txt", m); }, "inconsistent")));
         for (const char* suf : {"_v.bin", "_p.bin", "_m.txt", "_k.bin", "_q.txt"})
             std::remove((base + suf).c_str());
+    }
+    {
+        // tests/cpu/test_geometry.cpp, through the C++ layer
+        auto close = [](float a, float b) { return std::fabs(a - b) < 1.0e-6F; };
+        const auto p32 = bfa::default_positions(32);
+        EXPECT(p32.size() == 32 && (p32[0] == bfa::Vec3{0.0F, 0.0F, 0.0F}));
+        EXPECT(close(p32[7][0], 4.2F) && close(p32[8][1], 0.6F) && close(p32[31][1], 1.8F));
+        EXPECT((bfa::default_positions(64, 0.5F)[63] == bfa::Vec3{3.5F, 3.5F, 0.0F}));
+        const auto fr = bfa::channelized_frequencies();
+        EXPECT(fr.size() == 336 && fr.front() == 300'000'000.0F && fr.back() == 400'500'000.0F);
+        const auto b5 = bfa::default_beam_grid(5);
+        EXPECT(b5[0][0] == -0.04F && b5[2][0] == 0.0F && b5[4][0] == 0.04F);
+        EXPECT(bfa::default_beam_grid(10)[7][0] == 0.04F);
+        const float lambda = static_cast<float>(299792458.0 / 400'000'000.0);
+        const auto r32 = bfa::rectangular_beam_grid(32);
+        EXPECT(r32.size() == 32 && close(r32[1][0] - r32[0][0], lambda / (8.0F * 0.6F)));
+        EXPECT((bfa::centered_integer_range(4) == std::vector<long long>{-2, -1, 0, 1}));
+        const float du = lambda / (2.0F * 8.0F * 0.6F);
+        const auto s32 = bfa::select_fft_beam_centers_rectangular(du, 2.0F * du, 128, 8, 4);
+        EXPECT(s32.centers.size() == 128 && s32.n_u == 16 && s32.n_v == 8);
+        const auto s64 = bfa::select_fft_beam_centers_rectangular(du, du, 128, 8, 8);
+        EXPECT(s64.n_u == 12 && s64.n_v == 11 && s64.n_bank_u == 16 && s64.n_bank_v == 16);
+        EXPECT(bfa::fft_beam_grid(32, 128).size() == 128);
+        EXPECT(bfa::estimate_nbeams_hex_formula(0.6F, 8, 8, 108.0F, 74.0F, lambda) > 0);
+        const auto hx = bfa::generate_hex_targets_cropped_fov(128, 0.8F, 0.6F);
+        bool inside = hx.size() == 128;
+        for (const auto& t : hx)
+            inside = inside && t[0] * t[0] / 0.64F + t[1] * t[1] / 0.36F <= 1.0F + 1.0e-6F;
+        EXPECT(inside);
+        const std::string base = std::string("/tmp/bf_geom_") + std::to_string(getpid());
+        {
+            std::ofstream o(base + "_p.txt");
+            o << "# x, y, z in metres\n1.0, 2.0, 3.0\n4.0 5.0 6.0\n";
+        }
+        const auto lp = bfa::load_positions(base + "_p.txt", 2);
+        EXPECT((lp[1] == bfa::Vec3{4.0F, 5.0F, 6.0F}));
+        EXPECT((throws<std::runtime_error>([&] { bfa::load_positions(base + "_p.txt", 3); },
+                                           "expected 3")));
+        std::remove((base + "_p.txt").c_str());
+        EXPECT((throws<std::invalid_argument>([] { bfa::default_positions(48); }, "256")));
+        const auto w = bfa::generate_weights({1, 336, 32, 2}, p32, fr, bfa::default_beam_grid(2));
+        EXPECT(w.size() == 2 * 2 * 336 * 32 && w[0] == 1.0F && w[1] == 0.0F);
+        const auto t = bfa::generate_tiled_weights({1, 336, 32, 2}, p32, fr,
+                                                   bfa::default_beam_grid(2));
+        EXPECT(t.size() == 2 * 336 * 1 * 32 * 32);
+        // adaptive tracker: constructor checks of cpu_opt_beam_tracker.cpp:1084-1113
+        bfa::OptTrackerConfig bad;
+        bad.forgetting_factor = 1.5F;
+        EXPECT((throws<std::invalid_argument>(
+            [&] { bfa::OptBeamTracker tr({}, {}, {4, 336, 32, 1}, bad); }, "forgetting")));
+        bad = {};
+        bad.spatial_smoothing_subarray_size = 100;
+        EXPECT((throws<std::invalid_argument>(
+            [&] { bfa::OptBeamTracker tr({}, {}, {4, 336, 32, 1}, bad); }, "subarray")));
+        EXPECT((throws<std::invalid_argument>(
+            [] { bfa::OptBeamTracker tr({}, {}, {4, 336, 32, 2}, {}); }, "n_beams == 1")));
     }
     std::printf("%s (%d failures)\n", failures ? "FAIL" : "OK", failures);
     return failures ? 1 : 0;
