@@ -291,24 +291,22 @@ __global__ __launch_bounds__(kThreads) void opt_factor_kernel(
         cd rj[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) rj[v] = rowb[buf][j0 + v];
+        // Branch-free: the general update for every element, then selects
+        // for the pivot row, the pivot column and the pivot itself.
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int i = i0 + u;
-            const cd ci = colb[buf][i];
+            const bool row_k = i0 + u == k;
+            const cd ci = colb[buf][i0 + u];
             const cd cp{ci.re * p, ci.im * p};
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int j = j0 + v;
-                if (i != k && j != k) {
-                    a[u][v].re -= cp.re * rj[v].re - cp.im * rj[v].im;
-                    a[u][v].im -= cp.re * rj[v].im + cp.im * rj[v].re;
-                } else if (i == k && j != k) {
-                    a[u][v] = cd{rj[v].re * p, rj[v].im * p};
-                } else if (j == k && i != k) {
-                    a[u][v] = cd{-cp.re, -cp.im};
-                } else {
-                    a[u][v] = cd{p, 0.0};
-                }
+                const bool col_k = j0 + v == k;
+                const double gre = fma(-cp.re, rj[v].re, fma(cp.im, rj[v].im, a[u][v].re));
+                const double gim = fma(-cp.re, rj[v].im, fma(-cp.im, rj[v].re, a[u][v].im));
+                const double rre = rj[v].re * p, rim = rj[v].im * p;
+                const double ore = row_k ? (col_k ? p : rre) : (col_k ? -cp.re : gre);
+                const double oim = row_k ? (col_k ? 0.0 : rim) : (col_k ? -cp.im : gim);
+                a[u][v] = cd{ore, oim};
             }
         }
     }
@@ -360,12 +358,12 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
     return v;
 }
 
-// One pass of a wave over NC cells: acc_lane = sum_r op(a_r, Q[r][lane]),
-// the a_r broadcast from lane r.  BART: conj(a_r) R[r][lane] over the strict
-// upper triangle; else W[lane][r] a_r (Q holds W transposed).  Q is a
-// zero-padded 64 x 64 tile and a_r = 0 for r >= M, so rows are taken four
-// at a time (the four LDS reads issue together) up to M rounded up.
-template <int NC, bool BART>
+// One pass of a wave over NC cells: acc_lane = sum_r conj(a_r) R[r][lane]
+// over the strict upper triangle of R, a_r broadcast from this wave's LDS
+// steering vectors.  R is a zero-padded 64 x 64 tile and a_r = 0 for
+// r >= M, so rows are taken four at a time (the four LDS reads issue
+// together) up to M rounded up.
+template <int NC>
 __device__ __forceinline__ void scan_pass(const float2* __restrict__ Q,
                                           const float2* __restrict__ sa, int M, int lane,
                                           float2 (&acc)[NC]) {
@@ -380,19 +378,15 @@ __device__ __forceinline__ void scan_pass(const float2* __restrict__ Q,
             for (int j = 0; j < NC; ++j) {
                 const float2 av = sa[j * 64 + r];  // broadcast
                 const float ar = av.x, ai = av.y;
-                if (BART) {
-                    acc[j].x = fmaf(ai, q[u].y, fmaf(ar, q[u].x, acc[j].x));
-                    acc[j].y = fmaf(-ai, q[u].x, fmaf(ar, q[u].y, acc[j].y));
-                } else {
-                    acc[j].x = fmaf(-q[u].y, ai, fmaf(q[u].x, ar, acc[j].x));
-                    acc[j].y = fmaf(q[u].y, ar, fmaf(q[u].x, ai, acc[j].y));
-                }
+                acc[j].x = fmaf(ai, q[u].y, fmaf(ar, q[u].x, acc[j].x));
+                acc[j].y = fmaf(-ai, q[u].x, fmaf(ar, q[u].y, acc[j].y));
             }
         }
     }
 }
 
-template <int NC, bool BART>
+// Bartlett form: P = tr(R) + 2 Re sum_c y_c a_c (also Capon's fallback).
+template <int NC>
 __device__ void scan_cells(const float2* __restrict__ Q, float2* __restrict__ sa,
                            const OptSearch& s,
                            const OptGridArgs& g, int level, int c_begin, int c_end, int lane,
@@ -419,20 +413,12 @@ __device__ void scan_cells(const float2* __restrict__ Q, float2* __restrict__ sa
         }
         // the wave's own LDS writes precede its reads in order
         __builtin_amdgcn_wave_barrier();
-        scan_pass<NC, BART>(Q, sa, g.M, lane, acc);
+        scan_pass<NC>(Q, sa, g.M, lane, acc);
 #pragma unroll
         for (int j = 0; j < NC; ++j) {
-            float t;
-            if (BART)
-                t = acc[j].x * a[j].x - acc[j].y * a[j].y;  // Re(y_c a_c)
-            else
-                t = acc[j].x * acc[j].x + acc[j].y * acc[j].y;
+            float t = acc[j].x * a[j].x - acc[j].y * a[j].y;  // Re(y_c a_c)
             t = wave_sum(t);
-            float p;
-            if (BART)
-                p = tr + 2.0f * t;
-            else
-                p = t > 1.0e-12f ? 1.0f / t : 0.0f;
+            float p = tr + 2.0f * t;
             if (!isfinite(p)) p = 0.0f;
             if (lane == 0 && c0 + j < c_end) part[(size_t)(c0 + j) * F + f] = p;
         }
@@ -546,7 +532,7 @@ __global__ __launch_bounds__(kThreads) void opt_scan_kernel(
                                  g, level, c_begin, c_end, lane, wave, f, F, kw, trace64[f],
                            px, py, pz, part);
     else
-        scan_cells<NC, true>(Q, reinterpret_cast<float2*>(steer) + wave * NC * 64, s, g,
+        scan_cells<NC>(Q, reinterpret_cast<float2*>(steer) + wave * NC * 64, s, g,
                              level, c_begin, c_end, lane, wave, f, F, kw, tr, px, py, pz,
                          part);
 }
