@@ -68,21 +68,43 @@ __global__ __launch_bounds__(kThreads) void covariance_kernel(
     for (int i = 0; i < PER_WAVE; ++i) acc[i] = bf_int4v{0, 0, 0, 0};
     const size_t row_stride = (size_t)F * A;
     const uint8_t* xf = x + (size_t)f * A;
-    // Staging work: 64 snapshots x A antennas = 16 snapshot quads x A/4 antenna quads.
+    // Staging work: 64 snapshots x A antennas = 16 snapshot quads x A/4
+    // antenna quads, QPT per thread.  The dwords of the next 64-snapshot
+    // chunk are loaded into registers while this chunk's MFMAs run.
     constexpr int QUADS = 16 * (A / 4);
-    for (int k0 = 0; k0 < K; k0 += 64) {
-        for (int q = tid; q < QUADS; q += kThreads) {
+    constexpr int QPT = (QUADS + kThreads - 1) / kThreads;
+    unsigned nxt[QPT][4];
+    auto load_chunk = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < QPT; ++j) {
+            const int q = tid + j * kThreads;
             const int k4 = q / (A / 4);
             const int a4 = q - k4 * (A / 4);
-            unsigned d[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int kk = k0 + 4 * k4 + i;
-                d[i] = 0u;
-                if (kk < K && t0 + kk < T)
-                    d[i] = *reinterpret_cast<const unsigned*>(
+                nxt[j][i] = 0u;
+                if (q < QUADS && kk < K && t0 + kk < T)
+                    nxt[j][i] = *reinterpret_cast<const unsigned*>(
                         xf + (size_t)(t0 + kk) * row_stride + 4 * a4);
             }
+        }
+    };
+    load_chunk(0);
+    for (int k0 = 0; k0 < K; k0 += 64) {
+        unsigned cur[QPT][4];
+#pragma unroll
+        for (int j = 0; j < QPT; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cur[j][i] = nxt[j][i];
+        if (k0 + 64 < K) load_chunk(k0 + 64);
+#pragma unroll
+        for (int jq = 0; jq < QPT; ++jq) {
+            const int q = tid + jq * kThreads;
+            if (q >= QUADS) continue;
+            const int k4 = q / (A / 4);
+            const int a4 = q - k4 * (A / 4);
+            const unsigned* d = cur[jq];
             unsigned e[4];
             transpose4x4(d, e);
 #pragma unroll

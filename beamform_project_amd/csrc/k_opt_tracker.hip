@@ -570,11 +570,29 @@ __global__ __launch_bounds__(kDecideThreads) void opt_decide_kernel(
         }
     }
     __syncthreads();
-    if (tid != 0) return;
+    if (wave != 0) return;
+    // argmax with the reference's rule (strict >, so the first maximum):
+    // each lane scans its cells in increasing order, then the lanes combine
+    // preferring the larger value and, on a tie, the lower index
+    float bv = -__builtin_huge_valf();
+    int bi = n_cells;  // none yet
+    for (int i = lane; i < n_cells; i += 64)
+        if (bi == n_cells || P[i] > bv) {
+            bv = P[i];
+            bi = i;
+        }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const float ov = __shfl_xor(bv, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (oi < n_cells && (bi == n_cells || ov > bv || (ov == bv && oi < bi))) {
+            bv = ov;
+            bi = oi;
+        }
+    }
+    if (lane != 0) return;
     OptSearch s = *search;
-    int best = 0;
-    for (int i = 1; i < n_cells; ++i)
-        if (P[i] > P[best]) best = i;
+    const int best = bi;
     float dir[3];
     double st[3];
     cell_direction(g, s, level, best, dir, st);
