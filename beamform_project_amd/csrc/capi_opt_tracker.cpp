@@ -130,6 +130,9 @@ struct bf_opt_tracker {
 
     void ensure_dirs(size_t W) {
         if (W <= dirs_cap) return;
+        // a previous run may still write the old table
+        if (last_stream != nullptr || dirs_cap != 0)
+            hip_check(hipStreamSynchronize(last_stream), "hipStreamSynchronize");
         if (d_dirs) hip_check(hipFree(d_dirs), "hipFree");
         d_dirs = nullptr;
         d_dirs = dmalloc<float>(3 * W);
