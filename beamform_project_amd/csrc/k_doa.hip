@@ -9,7 +9,8 @@
 //   2. fold_scan_kernel (FP64): R_block = Rs/(K L), diagonal load
 //      eps = load * tr(R_block) / M added before the fold (:244-253,
 //      :1874-1890), R = alpha R + (1-alpha) R_block (first update copies),
-//      Cholesky R = L L^H (:398-421), Q = R^-1 = L^-H L^-1, and the Bartlett
+//      Q = R^-1 by register-blocked Gauss-Jordan (gauss_jordan.hpp; its
+//      pivots are the Cholesky pivots of :398-421), and the Bartlett
 //      a^H R a and Capon 1/(a^H Q a) scans over n_theta angles of a uniform
 //      linear array, steering a_m = exp(-j k x_m sin(theta)) (:96-106).
 //      For a ULA a^H Q a = sum_d q_d exp(j d k dx sin(theta)) with q_d the
@@ -24,6 +25,7 @@
 // "component x 16 snapshots" rows of a transposed LDS tile.  Then
 // Re R = (G_rr + G_ii)/256 and Im R = (G_ir - G_ri)/256 exactly.
 #include "bf_doa.hpp"
+#include "gauss_jordan.hpp"
 
 namespace bfhip {
 namespace {
@@ -178,9 +180,6 @@ struct cd {
 __device__ __forceinline__ cd cmul(cd a, cd b) {
     return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
-__device__ __forceinline__ cd cmulc(cd a, cd b) {  // conj(a) * b
-    return cd{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
-}
 
 // One workgroup per channel.  buf0/buf1: M x M complex double in LDS.
 __global__ __launch_bounds__(kThreads) void fold_scan_kernel(
@@ -195,6 +194,7 @@ __global__ __launch_bounds__(kThreads) void fold_scan_kernel(
     cd* qc = qb + M;            // [M] Capon diagonal sums
     __shared__ double s_eps;
     __shared__ int s_ok;
+    __shared__ GjShared<cd> gj;
     const int f = blockIdx.x;
     const int tid = threadIdx.x;
     const int MM = M * M;
@@ -237,65 +237,27 @@ __global__ __launch_bounds__(kThreads) void fold_scan_kernel(
         qb[d] = s;
     }
     __syncthreads();
-    // In-place right-looking Cholesky of buf0 (lower triangle).
-    for (int j = 0; j < M; ++j) {
-        if (tid == 0) {
-            const double djj = buf0[j * M + j].re;
-            if (!(djj > 0.0)) s_ok = 0;
-            buf0[j * M + j] = cd{djj > 0.0 ? sqrt(djj) : 1.0, 0.0};
-        }
-        __syncthreads();
-        const double dinv = 1.0 / buf0[j * M + j].re;
-        for (int i = j + 1 + tid; i < M; i += kThreads) {
-            cd v = buf0[i * M + j];
-            buf0[i * M + j] = cd{v.re * dinv, v.im * dinv};
-        }
-        __syncthreads();
-        const int n = M - j - 1;
-        for (int e = tid; e < n * n; e += kThreads) {
-            const int i = j + 1 + e / n;
-            const int k = j + 1 + e % n;
-            if (k <= i) {
-                const cd lik = buf0[i * M + j];
-                const cd lkj = buf0[k * M + j];
-                // A[i][k] -= L[i][j] conj(L[k][j])
-                const cd p{lik.re * lkj.re + lik.im * lkj.im, lik.im * lkj.re - lik.re * lkj.im};
-                buf0[i * M + k].re -= p.re;
-                buf0[i * M + k].im -= p.im;
-            }
+    // Q = R^-1 in place by register-blocked Gauss-Jordan; its pivots are
+    // the Cholesky pivots, so R fails the factorisation exactly when a
+    // pivot is not positive.
+    {
+        cd blk[4][4];
+        gj_load(blk, reinterpret_cast<const double*>(buf0), M, tid);
+        __syncthreads();  // every block read before Q overwrites buf0
+        const bool ok_gj = gj_invert(blk, M, 0.0, gj, tid);
+        if (tid == 0) s_ok = ok_gj ? 1 : 0;
+        if (ok_gj) {
+            const int i0 = 4 * (tid >> 4), j0 = 4 * (tid & 15);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (i0 + u < M && j0 + v < M) buf0[(i0 + u) * M + j0 + v] = blk[u][v];
         }
         __syncthreads();
     }
     const int ok = s_ok;
     if (tid == 0) chol_ok[f] = ok;
-    // Linv (lower) into buf1, one column per thread.
-    for (int j = tid; j < M; j += kThreads) {
-        for (int i = 0; i < j; ++i) buf1[i * M + j] = cd{0.0, 0.0};
-        for (int i = j; i < M; ++i) {
-            cd s{i == j ? 1.0 : 0.0, 0.0};
-            for (int k = j; k < i; ++k) {
-                const cd p = cmul(buf0[i * M + k], buf1[k * M + j]);
-                s.re -= p.re;
-                s.im -= p.im;
-            }
-            const double inv = 1.0 / buf0[i * M + i].re;
-            buf1[i * M + j] = cd{s.re * inv, s.im * inv};
-        }
-    }
-    __syncthreads();
-    // Q = Linv^H Linv into buf0.
-    for (int e = tid; e < MM; e += kThreads) {
-        const int r = e / M, c = e % M;
-        const int k0 = r > c ? r : c;
-        cd s{0.0, 0.0};
-        for (int k = k0; k < M; ++k) {
-            const cd p = cmulc(buf1[k * M + r], buf1[k * M + c]);
-            s.re += p.re;
-            s.im += p.im;
-        }
-        buf0[e] = s;
-    }
-    __syncthreads();
     for (int d = tid; d < M; d += kThreads) {
         cd s{0.0, 0.0};
         for (int c = 0; c + d < M; ++c) {

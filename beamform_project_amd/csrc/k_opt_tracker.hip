@@ -38,6 +38,7 @@
 #include <algorithm>
 
 #include "bf_doa.hpp"
+#include "gauss_jordan.hpp"
 
 namespace bfhip {
 namespace {
@@ -246,8 +247,7 @@ __global__ __launch_bounds__(kThreads) void opt_factor_kernel(
     const double* __restrict__ state, int M, double2* __restrict__ scan64,
     double* __restrict__ trace64, float2* __restrict__ scanmat, float* __restrict__ trace_out,
     int* __restrict__ mode_out) {
-    __shared__ cd rowb[2][64];
-    __shared__ cd colb[2][64];
+    __shared__ GjShared<cd> sh;
     __shared__ double s_diag[64];
     const int f = blockIdx.x;
     const int tid = threadIdx.x;
@@ -255,61 +255,8 @@ __global__ __launch_bounds__(kThreads) void opt_factor_kernel(
     const int MM = M * M;
     const double* R = state + (size_t)f * MM * 2;
     cd a[4][4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int i = i0 + u, j = j0 + v;
-            if (i < M && j < M)
-                a[u][v] = cd{R[2 * (i * M + j)], R[2 * (i * M + j) + 1]};
-            else
-                a[u][v] = cd{i == j ? 1.0 : 0.0, 0.0};
-        }
-    bool ok = true;
-    for (int k = 0; k < 64; ++k) {
-        const int buf = k & 1;
-        // static register indices only (a runtime index would spill a[][])
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (i0 + u == k) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) rowb[buf][j0 + v] = a[u][v];
-            }
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-            if (j0 + v == k) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) colb[buf][i0 + u] = a[u][v];
-            }
-        __syncthreads();
-        const double piv = rowb[buf][k].re;
-        if (k < M && !(piv > 1e-12)) {  // uniform: every thread reads the same pivot
-            ok = false;
-            break;
-        }
-        const double p = 1.0 / piv;
-        cd rj[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) rj[v] = rowb[buf][j0 + v];
-        // Branch-free: the general update for every element, then selects
-        // for the pivot row, the pivot column and the pivot itself.
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const bool row_k = i0 + u == k;
-            const cd ci = colb[buf][i0 + u];
-            const cd cp{ci.re * p, ci.im * p};
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const bool col_k = j0 + v == k;
-                const double gre = fma(-cp.re, rj[v].re, fma(cp.im, rj[v].im, a[u][v].re));
-                const double gim = fma(-cp.re, rj[v].im, fma(-cp.im, rj[v].re, a[u][v].im));
-                const double rre = rj[v].re * p, rim = rj[v].im * p;
-                const double ore = row_k ? (col_k ? p : rre) : (col_k ? -cp.re : gre);
-                const double oim = row_k ? (col_k ? 0.0 : rim) : (col_k ? -cp.im : gim);
-                a[u][v] = cd{ore, oim};
-            }
-        }
-    }
+    gj_load(a, R, M, tid);
+    const bool ok = gj_invert(a, M, 1e-12, sh, tid);
     if (!ok) {
         float2* Q = scanmat + (size_t)f * MM;
         for (int e = tid; e < MM; e += kThreads) {
