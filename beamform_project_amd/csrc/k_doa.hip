@@ -45,14 +45,28 @@ __device__ __forceinline__ void transpose4x4(const unsigned d[4], unsigned e[4])
     e[3] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
 }
 
-// One workgroup per channel; A in {32, 64, 128, 256} -> 2A components.
+// Tile t of the upper triangle (I <= J) of an n x n tile grid, row by row.
+__device__ __forceinline__ void upper_tile(int t, int n, int* I, int* J) {
+    int i = 0;
+    while (t >= n - i) {
+        t -= n - i;
+        ++i;
+    }
+    *I = i;
+    *J = i + t;
+}
+
+// One workgroup per channel; A in {32, 64} -> 2A components.  G is
+// symmetric, so only the tiles on and above the diagonal are accumulated;
+// every entry the covariance reads for r <= c lies there, and r > c is the
+// conjugate of (c, r).
 template <int A>
 __global__ __launch_bounds__(kThreads) void covariance_kernel(
     const uint8_t* __restrict__ x, int T, int F, int t0, int K, int M,
     int* __restrict__ rs_out /* [F][M][M][2] */, int* __restrict__ r_full /* [F][A][A][2] or null */) {
     constexpr int C = 2 * A;         // components
     constexpr int NTILE = C / 16;    // tile rows/cols of G
-    constexpr int TILES = NTILE * NTILE;
+    constexpr int TILES = NTILE * (NTILE + 1) / 2;  // upper triangle
     constexpr int PER_WAVE = (TILES + 3) / 4;
     __shared__ __attribute__((aligned(16))) unsigned char tr[C * kTStride];
     __shared__ int g[C * (C + 1)];  // G, row stride C+1
@@ -64,10 +78,14 @@ __global__ __launch_bounds__(kThreads) void covariance_kernel(
     if (f >= F) return;  // whole workgroup
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     bf_int4v acc[PER_WAVE];
+    int tI[PER_WAVE], tJ[PER_WAVE];  // this wave's tiles, wave-uniform
 #pragma unroll
-    for (int i = 0; i < PER_WAVE; ++i) acc[i] = bf_int4v{0, 0, 0, 0};
+    for (int i = 0; i < PER_WAVE; ++i) {
+        acc[i] = bf_int4v{0, 0, 0, 0};
+        upper_tile(min(wave * PER_WAVE + i, TILES - 1), NTILE, &tI[i], &tJ[i]);
+    }
     const size_t row_stride = (size_t)F * A;
     const uint8_t* xf = x + (size_t)f * A;
     // Staging work: 64 snapshots x A antennas = 16 snapshot quads x A/4
@@ -125,8 +143,7 @@ __global__ __launch_bounds__(kThreads) void covariance_kernel(
         for (int i = 0; i < PER_WAVE; ++i) {
             const int tile = wave * PER_WAVE + i;
             if (tile < TILES) {
-                const int I = tile / NTILE;
-                const int J = tile - I * NTILE;
+                const int I = tI[i], J = tJ[i];
                 const bf_int4v av =
                     *reinterpret_cast<const bf_int4v*>(tr + (16 * I + rc) * kTStride + 16 * kg);
                 const bf_int4v bv =
@@ -141,8 +158,7 @@ __global__ __launch_bounds__(kThreads) void covariance_kernel(
     for (int i = 0; i < PER_WAVE; ++i) {
         const int tile = wave * PER_WAVE + i;
         if (tile < TILES) {
-            const int I = tile / NTILE;
-            const int J = tile - I * NTILE;
+            const int I = tI[i], J = tJ[i];
 #pragma unroll
             for (int r = 0; r < 4; ++r)
                 g[(16 * I + 4 * (lane >> 4) + r) * (C + 1) + 16 * J + (lane & 15)] = acc[i][r];
@@ -152,24 +168,26 @@ __global__ __launch_bounds__(kThreads) void covariance_kernel(
     if (r_full != nullptr) {
         for (int e = tid; e < A * A; e += kThreads) {
             const int r = e / A, c = e - (e / A) * A;
-            const int gre = g[(2 * r) * (C + 1) + 2 * c] + g[(2 * r + 1) * (C + 1) + 2 * c + 1];
-            const int gim = g[(2 * r + 1) * (C + 1) + 2 * c] - g[(2 * r) * (C + 1) + 2 * c + 1];
+            const int a = r <= c ? r : c, b = r <= c ? c : r;  // upper entry
+            const int gre = g[(2 * a) * (C + 1) + 2 * b] + g[(2 * a + 1) * (C + 1) + 2 * b + 1];
+            const int gim = g[(2 * a + 1) * (C + 1) + 2 * b] - g[(2 * a) * (C + 1) + 2 * b + 1];
             r_full[(((size_t)f * A + r) * A + c) * 2] = gre / 256;
-            r_full[(((size_t)f * A + r) * A + c) * 2 + 1] = gim / 256;
+            r_full[(((size_t)f * A + r) * A + c) * 2 + 1] = (r <= c ? gim : -gim) / 256;
         }
     }
     const int L = A - M + 1;
     for (int e = tid; e < M * M; e += kThreads) {
         const int r = e / M, c = e - (e / M) * M;
+        const int a = r <= c ? r : c, b = r <= c ? c : r;  // upper entry
         int sre = 0, sim = 0;
         for (int l = 0; l < L; ++l) {
-            const int rr = l + r, cc = l + c;
+            const int rr = l + a, cc = l + b;
             // every G entry is a multiple of 256 (16 x 16 scaling)
             sre += (g[(2 * rr) * (C + 1) + 2 * cc] + g[(2 * rr + 1) * (C + 1) + 2 * cc + 1]) / 256;
             sim += (g[(2 * rr + 1) * (C + 1) + 2 * cc] - g[(2 * rr) * (C + 1) + 2 * cc + 1]) / 256;
         }
         rs_out[(((size_t)f * M + r) * M + c) * 2] = sre;
-        rs_out[(((size_t)f * M + r) * M + c) * 2 + 1] = sim;
+        rs_out[(((size_t)f * M + r) * M + c) * 2 + 1] = r <= c ? sim : -sim;
     }
 }
 
