@@ -306,19 +306,21 @@ __global__ __launch_bounds__(kThreads) void fold_scan_kernel(
 }
 
 // Broadband spectra: sum over channels.  A workgroup covers 64 angles; its
-// four waves each sum a quarter of the channels (f = part, part + 4, ...)
-// and the four partial sums are added in a fixed order.
-__global__ __launch_bounds__(kThreads) void broadband_kernel(
+// kBbParts waves each sum every kBbParts-th channel (f = part, part +
+// kBbParts, ...) and the partial sums are added in part order.
+constexpr int kBbParts = 16;
+
+__global__ __launch_bounds__(64 * kBbParts) void broadband_kernel(
     const double* __restrict__ capon, const double* __restrict__ bartlett, int F, int n_theta,
     double* __restrict__ broadband /* [2][n] */) {
-    __shared__ double part_c[4][64];
-    __shared__ double part_b[4][64];
+    __shared__ double part_c[kBbParts][64];
+    __shared__ double part_b[kBbParts][64];
     const int ql = threadIdx.x & 63;
     const int part = threadIdx.x >> 6;
     const int q = blockIdx.x * 64 + ql;
     double sc = 0.0, sb = 0.0;
     if (q < n_theta) {
-        for (int f = part; f < F; f += 4) {
+        for (int f = part; f < F; f += kBbParts) {
             sc += capon[(size_t)f * n_theta + q];
             sb += bartlett[(size_t)f * n_theta + q];
         }
@@ -327,13 +329,18 @@ __global__ __launch_bounds__(kThreads) void broadband_kernel(
     part_b[part][ql] = sb;
     __syncthreads();
     if (part == 0 && q < n_theta) {
-        broadband[q] = ((part_c[0][ql] + part_c[1][ql]) + part_c[2][ql]) + part_c[3][ql];
-        broadband[n_theta + q] =
-            ((part_b[0][ql] + part_b[1][ql]) + part_b[2][ql]) + part_b[3][ql];
+        double c = part_c[0][ql], b = part_b[0][ql];
+        for (int i = 1; i < kBbParts; ++i) {
+            c += part_c[i][ql];
+            b += part_b[i][ql];
+        }
+        broadband[q] = c;
+        broadband[n_theta + q] = b;
     }
 }
 
 constexpr int kMaxCandidates = 4096;
+constexpr int kPeaksLds = 4096;  // spectra up to this many angles staged in LDS
 
 // Peaks of one broadband spectrum (workgroup 0: Capon, 1: Bartlett): local
 // maxima (P[q] > P[q-1] and P[q] >= P[q+1]) are found in parallel and
@@ -347,7 +354,17 @@ __global__ __launch_bounds__(kThreads) void peaks_kernel(
     __shared__ int cand[kMaxCandidates];
     __shared__ int wave_count[kThreads / 64];
     __shared__ int total;
-    const double* P = broadband + (size_t)blockIdx.x * n_theta;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    // the spectrum, staged in LDS when it fits (the serial refinement below
+    // reads it three times per candidate)
+    const double* Pg = broadband + (size_t)blockIdx.x * n_theta;
+    const double* P = Pg;
+    if (n_theta <= kPeaksLds) {
+        double* Ps = reinterpret_cast<double*>(smem);
+        for (int q = threadIdx.x; q < n_theta; q += kThreads) Ps[q] = Pg[q];
+        __syncthreads();
+        P = Ps;
+    }
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -441,9 +458,11 @@ void launch_doa_fold_scan(hipStream_t stream, const int* rs, double* state, bool
 void launch_doa_estimate(hipStream_t stream, const double* capon, const double* bartlett,
                          size_t F, size_t n_theta, const double* theta, size_t max_peaks,
                          double* broadband, double* peaks, int* n_peaks) {
-    hipLaunchKernelGGL(broadband_kernel, dim3((unsigned)ceil_div(n_theta, 64)), dim3(kThreads),
-                       0, stream, capon, bartlett, (int)F, (int)n_theta, broadband);
-    hipLaunchKernelGGL(peaks_kernel, dim3(2), dim3(kThreads), 0, stream, broadband,
+    hipLaunchKernelGGL(broadband_kernel, dim3((unsigned)ceil_div(n_theta, 64)),
+                       dim3(64 * kBbParts), 0, stream, capon, bartlett, (int)F, (int)n_theta,
+                       broadband);
+    const size_t smem = n_theta <= (size_t)kPeaksLds ? n_theta * sizeof(double) : 0;
+    hipLaunchKernelGGL(peaks_kernel, dim3(2), dim3(kThreads), smem, stream, broadband,
                        (int)n_theta, theta, (int)max_peaks, peaks, n_peaks);
     hip_check(hipGetLastError(), "estimate kernels launch");
 }
