@@ -1,10 +1,11 @@
 // gauss_jordan.hpp -- in-place inverse of a Hermitian positive-definite
 // complex FP64 matrix (M <= 64) by Gauss-Jordan elimination without
-// pivoting, for one 256-thread workgroup.  Thread t holds the 4 x 4 block
-// of rows 4 (t / 16) .. and columns 4 (t % 16) .. in registers (identity
-// outside M x M).  Per pivot k the owners publish row k and column k to LDS
-// (double-buffered, so one barrier per pivot) and every thread updates its
-// block without branches.  Without pivoting on a Hermitian positive-definite
+// pivoting, for one workgroup of (64 / BS)^2 threads.  Thread t holds the
+// BS x BS block of rows BS (t / (64 / BS)) .. and columns BS (t % (64 / BS))
+// .. in registers (identity outside M x M); BS = 2 (1024 threads) keeps each
+// wave's chain per pivot short.  Per pivot k the owners publish row k and
+// column k to LDS (double-buffered, so one barrier per pivot) and every
+// thread updates its block without branches.  Without pivoting on a Hermitian positive-definite
 // matrix the k-th pivot equals the Cholesky pivot L_kk^2, so a factorisation
 // test on it carries over unchanged.
 #pragma once
@@ -21,13 +22,18 @@ struct GjShared {
 
 // Loads the block of a row-major M x M matrix (re, im at 2 (i M + j)) held
 // in `src` (global or LDS) into registers.
-template <typename Cd>
-__device__ __forceinline__ void gj_load(Cd (&a)[4][4], const double* src, int M, int tid) {
-    const int i0 = 4 * (tid >> 4), j0 = 4 * (tid & 15);
+template <int BS>
+__device__ __forceinline__ int gj_row0(int tid) { return BS * (tid / (64 / BS)); }
+template <int BS>
+__device__ __forceinline__ int gj_col0(int tid) { return BS * (tid % (64 / BS)); }
+
+template <int BS, typename Cd>
+__device__ __forceinline__ void gj_load(Cd (&a)[BS][BS], const double* src, int M, int tid) {
+    const int i0 = gj_row0<BS>(tid), j0 = gj_col0<BS>(tid);
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < BS; ++u)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
+        for (int v = 0; v < BS; ++v) {
             const int i = i0 + u, j = j0 + v;
             if (i < M && j < M)
                 a[u][v] = Cd{src[2 * (i * M + j)], src[2 * (i * M + j) + 1]};
@@ -38,39 +44,39 @@ __device__ __forceinline__ void gj_load(Cd (&a)[4][4], const double* src, int M,
 
 // Inverts in place; returns false (uniformly) at the first pivot k < M that
 // is not above `floor`, leaving the block unspecified.
-template <typename Cd>
-__device__ bool gj_invert(Cd (&a)[4][4], int M, double floor, GjShared<Cd>& sh, int tid) {
-    const int i0 = 4 * (tid >> 4), j0 = 4 * (tid & 15);
+template <int BS, typename Cd>
+__device__ bool gj_invert(Cd (&a)[BS][BS], int M, double floor, GjShared<Cd>& sh, int tid) {
+    const int i0 = gj_row0<BS>(tid), j0 = gj_col0<BS>(tid);
     // pivots beyond M act on the identity padding only: skipped
     for (int k = 0; k < M; ++k) {
         const int buf = k & 1;
         // static register indices only (a runtime index would spill a[][])
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < BS; ++u)
             if (i0 + u == k) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) sh.row[buf][j0 + v] = a[u][v];
+                for (int v = 0; v < BS; ++v) sh.row[buf][j0 + v] = a[u][v];
             }
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
+        for (int v = 0; v < BS; ++v)
             if (j0 + v == k) {
 #pragma unroll
-                for (int u = 0; u < 4; ++u) sh.col[buf][i0 + u] = a[u][v];
+                for (int u = 0; u < BS; ++u) sh.col[buf][i0 + u] = a[u][v];
             }
         __syncthreads();
         const double piv = sh.row[buf][k].re;
         if (!(piv > floor)) return false;  // uniform: every thread reads the same pivot
         const double p = 1.0 / piv;
-        Cd rj[4];
+        Cd rj[BS];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) rj[v] = sh.row[buf][j0 + v];
+        for (int v = 0; v < BS; ++v) rj[v] = sh.row[buf][j0 + v];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < BS; ++u) {
             const bool row_k = i0 + u == k;
             const Cd ci = sh.col[buf][i0 + u];
             const Cd cp{ci.re * p, ci.im * p};
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
+            for (int v = 0; v < BS; ++v) {
                 const bool col_k = j0 + v == k;
                 // a_ij - a_ik a_kj / a_kk, or the pivot row, column or pivot
                 const double gre = fma(-cp.re, rj[v].re, fma(cp.im, rj[v].im, a[u][v].re));

@@ -199,8 +199,12 @@ __device__ __forceinline__ cd cmul(cd a, cd b) {
     return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
-// One workgroup per channel.  buf0/buf1: M x M complex double in LDS.
-__global__ __launch_bounds__(kThreads) void fold_scan_kernel(
+// One workgroup of 256 threads per channel (4 x 4 Gauss-Jordan blocks; 2 x 2
+// blocks on 1024 threads measured slower: 16-wave barriers, twice the LDS reads).
+// buf0/buf1: M x M complex double in LDS.
+constexpr int kGjBlock = 4;
+constexpr int kFoldThreads = (64 / kGjBlock) * (64 / kGjBlock);  // 256
+__global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
     const int* __restrict__ rs, double* __restrict__ state /* [F][M][M][2] */, int first,
     int M, int K, int L, double alpha, double load, const float* __restrict__ freqs,
     double x_step, int n_theta, const double* __restrict__ sin_theta,
@@ -218,7 +222,7 @@ __global__ __launch_bounds__(kThreads) void fold_scan_kernel(
     const int MM = M * M;
     const double scale = 1.0 / ((double)K * (double)L);
     // R_block = Rs / (K L)
-    for (int e = tid; e < MM; e += kThreads) {
+    for (int e = tid; e < MM; e += kFoldThreads) {
         const int* p = rs + ((size_t)f * MM + e) * 2;
         buf0[e] = cd{(double)p[0] * scale, (double)p[1] * scale};
     }
@@ -232,7 +236,7 @@ __global__ __launch_bounds__(kThreads) void fold_scan_kernel(
     __syncthreads();
     // diagonal load, then fold into the carried state
     double* st = state + (size_t)f * MM * 2;
-    for (int e = tid; e < MM; e += kThreads) {
+    for (int e = tid; e < MM; e += kFoldThreads) {
         cd v = buf0[e];
         if (e / M == e % M) v.re += s_eps;
         if (!first) {
@@ -245,7 +249,7 @@ __global__ __launch_bounds__(kThreads) void fold_scan_kernel(
     }
     __syncthreads();
     // Bartlett diagonal sums of R: qb[d] = sum_c R[c+d][c]
-    for (int d = tid; d < M; d += kThreads) {
+    for (int d = tid; d < M; d += kFoldThreads) {
         cd s{0.0, 0.0};
         for (int c = 0; c + d < M; ++c) {
             const cd v = buf0[(c + d) * M + c];
@@ -259,24 +263,24 @@ __global__ __launch_bounds__(kThreads) void fold_scan_kernel(
     // the Cholesky pivots, so R fails the factorisation exactly when a
     // pivot is not positive.
     {
-        cd blk[4][4];
-        gj_load(blk, reinterpret_cast<const double*>(buf0), M, tid);
+        cd blk[kGjBlock][kGjBlock];
+        gj_load<kGjBlock>(blk, reinterpret_cast<const double*>(buf0), M, tid);
         __syncthreads();  // every block read before Q overwrites buf0
-        const bool ok_gj = gj_invert(blk, M, 0.0, gj, tid);
+        const bool ok_gj = gj_invert<kGjBlock>(blk, M, 0.0, gj, tid);
         if (tid == 0) s_ok = ok_gj ? 1 : 0;
         if (ok_gj) {
-            const int i0 = 4 * (tid >> 4), j0 = 4 * (tid & 15);
+            const int i0 = gj_row0<kGjBlock>(tid), j0 = gj_col0<kGjBlock>(tid);
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < kGjBlock; ++u)
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
+                for (int v = 0; v < kGjBlock; ++v)
                     if (i0 + u < M && j0 + v < M) buf0[(i0 + u) * M + j0 + v] = blk[u][v];
         }
         __syncthreads();
     }
     const int ok = s_ok;
     if (tid == 0) chol_ok[f] = ok;
-    for (int d = tid; d < M; d += kThreads) {
+    for (int d = tid; d < M; d += kFoldThreads) {
         cd s{0.0, 0.0};
         for (int c = 0; c + d < M; ++c) {
             const cd v = buf0[(c + d) * M + c];
@@ -288,7 +292,7 @@ __global__ __launch_bounds__(kThreads) void fold_scan_kernel(
     __syncthreads();
     // a^H X a = q_0 + 2 Re sum_{d>0} q_d e^{j d phi}, phi = k dx sin(theta).
     const double kw = kTwoPi * (double)freqs[f] / kSpeedOfLight;
-    for (int q = tid; q < n_theta; q += kThreads) {
+    for (int q = tid; q < n_theta; q += kFoldThreads) {
         const double phi = kw * x_step * sin_theta[q];
         double sn, cs;
         sincos(phi, &sn, &cs);
@@ -449,7 +453,7 @@ void launch_doa_fold_scan(hipStream_t stream, const int* rs, double* state, bool
     hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(fold_scan_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
               "hipFuncSetAttribute");
-    hipLaunchKernelGGL(fold_scan_kernel, dim3((unsigned)F), dim3(kThreads), smem, stream, rs,
+    hipLaunchKernelGGL(fold_scan_kernel, dim3((unsigned)F), dim3(kFoldThreads), smem, stream, rs,
                        state, first ? 1 : 0, (int)M, (int)K, (int)L, alpha, load, freqs, x_step,
                        (int)n_theta, sin_theta, capon, bartlett, chol_ok);
     hip_check(hipGetLastError(), "fold_scan_kernel launch");

@@ -231,6 +231,8 @@ __global__ __launch_bounds__(kThreads) void opt_fold_kernel(
 }
 
 // ---------------------------------------------------------------- factor
+constexpr int kGjBlock = 4;                                     // 4 x 4 blocks
+constexpr int kFactorThreads = (64 / kGjBlock) * (64 / kGjBlock);  // 256
 // Capon operand: Q = R^-1 of the folded state by in-place Gauss-Jordan
 // elimination in FP64, one workgroup per channel.  Thread t holds the 4 x 4
 // block (rows 4 (t / 16) .., columns 4 (t % 16) ..) of a 64 x 64 matrix in
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(kThreads) void opt_fold_kernel(
 // Q is written in full (FP64) with its trace; the scan evaluates
 // 1 / (tr Q + 2 Re sum_{r<c} conj(a_r) Q_rc a_c) = 1 / (a^H R^-1 a) in
 // FP64, where the large entries of R^-1 cancel.
-__global__ __launch_bounds__(kThreads) void opt_factor_kernel(
+__global__ __launch_bounds__(kFactorThreads) void opt_factor_kernel(
     const double* __restrict__ state, int M, double2* __restrict__ scan64,
     double* __restrict__ trace64, float2* __restrict__ scanmat, float* __restrict__ trace_out,
     int* __restrict__ mode_out) {
@@ -251,15 +253,15 @@ __global__ __launch_bounds__(kThreads) void opt_factor_kernel(
     __shared__ double s_diag[64];
     const int f = blockIdx.x;
     const int tid = threadIdx.x;
-    const int i0 = 4 * (tid >> 4), j0 = 4 * (tid & 15);
+    const int i0 = gj_row0<kGjBlock>(tid), j0 = gj_col0<kGjBlock>(tid);
     const int MM = M * M;
     const double* R = state + (size_t)f * MM * 2;
-    cd a[4][4];
-    gj_load(a, R, M, tid);
-    const bool ok = gj_invert(a, M, 1e-12, sh, tid);
+    cd a[kGjBlock][kGjBlock];
+    gj_load<kGjBlock>(a, R, M, tid);
+    const bool ok = gj_invert<kGjBlock>(a, M, 1e-12, sh, tid);
     if (!ok) {
         float2* Q = scanmat + (size_t)f * MM;
-        for (int e = tid; e < MM; e += kThreads) {
+        for (int e = tid; e < MM; e += kFactorThreads) {
             const int r = e / M, c = e - (e / M) * M;
             Q[e] = r < c ? make_float2((float)R[2 * e], (float)R[2 * e + 1])
                          : make_float2(0.0f, 0.0f);
@@ -274,9 +276,9 @@ __global__ __launch_bounds__(kThreads) void opt_factor_kernel(
     }
     double2* Q = scan64 + (size_t)f * MM;
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < kGjBlock; ++u)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
+        for (int v = 0; v < kGjBlock; ++v) {
             const int i = i0 + u, j = j0 + v;
             if (i < M && j < M) {
                 Q[i * M + j] = make_double2(a[u][v].re, a[u][v].im);
@@ -592,7 +594,7 @@ void launch_opt_fold(hipStream_t stream, const int* rs, double* state, bool firs
                        lambda, scanmat, trace, mode);
     hip_check(hipGetLastError(), "opt_fold_kernel launch");
     if (!capon) return;
-    hipLaunchKernelGGL(opt_factor_kernel, dim3((unsigned)F), dim3(kThreads), 0, stream, state,
+    hipLaunchKernelGGL(opt_factor_kernel, dim3((unsigned)F), dim3(kFactorThreads), 0, stream, state,
                        (int)M, scan64, trace64, scanmat, trace, mode);
     hip_check(hipGetLastError(), "opt_factor_kernel launch");
 }
