@@ -16,6 +16,8 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <cmath>
+#include <random>
 #include <string>
 #include <sys/stat.h>
 
@@ -435,6 +437,74 @@ bf_status bf_read_packed_shard(const char* payload_path, const char* metadata_pa
         read_exact(str(loss_mask_path, "loss_mask_path"), loss_mask, mask_bytes);
         validate_shard_payload(s, loss_mask, payload_bytes, mask_bytes, d);
         *shard_out = s;
+    });
+}
+
+bf_status bf_validate_packed_shard(const bf_shard* shard, const uint8_t* payload,
+                                   size_t payload_bytes, const uint8_t* loss_mask,
+                                   size_t loss_mask_bytes, const bf_dims* dims) {
+    return guarded([&] {
+        if (payload == nullptr && payload_bytes != 0) invalid("payload must not be null");
+        if (loss_mask == nullptr && loss_mask_bytes != 0) invalid("loss_mask must not be null");
+        // synthetic_data.cpp:139-153
+        validate_shard_payload(deref(shard), loss_mask, payload_bytes, loss_mask_bytes,
+                               deref_dims(dims));
+    });
+}
+
+bf_status bf_validate_packed_shards(const bf_shard* shards, const uint8_t* const* payloads,
+                                    size_t payload_bytes, const uint8_t* const* loss_masks,
+                                    size_t loss_mask_bytes, const bf_dims* dims) {
+    return guarded([&] {
+        if (shards == nullptr || payloads == nullptr || loss_masks == nullptr)
+            invalid("shard arrays must not be null");
+        const bf_dims& d = deref_dims(dims);
+        validate_dimensions(d);
+        // synthetic_data.cpp:155-171
+        for (size_t i = 0; i < kFrequencyShardCount; ++i) {
+            if (payloads[i] == nullptr || loss_masks[i] == nullptr)
+                invalid("shard buffers must not be null");
+            validate_shard_payload(shards[i], loss_masks[i], payload_bytes, loss_mask_bytes, d);
+            if (shards[i].shard_id != i) invalid("shard IDs must identify their array slot");
+        }
+        if (shards[0].absolute_frequency_start == shards[1].absolute_frequency_start)
+            invalid("the two shards must have distinct frequency origins");
+        if (shards[0].loss_mask_id == shards[1].loss_mask_id)
+            invalid("the two shards must have independent loss masks");
+    });
+}
+
+bf_status bf_make_loss_mask(const bf_dims* dims, uint32_t seed, float loss_probability,
+                            uint8_t* out, size_t n) {
+    return guarded([&] {
+        const bf_dims& d = deref_dims(dims);
+        validate_dimensions(d);
+        // make_loss_mask (synthetic_data.cpp:121-137): the same engine and
+        // distribution of the C++ standard library, one draw per frame
+        if (!std::isfinite(loss_probability) || loss_probability < 0.0f ||
+            loss_probability > 1.0f)
+            invalid("loss probability must be in [0, 1]");
+        if (out == nullptr) invalid("out must not be null");
+        if (n < d.n_time * d.n_freq) invalid("loss mask output is smaller than dimensions");
+        std::mt19937 random(seed);
+        std::bernoulli_distribution lost(loss_probability);
+        for (size_t i = 0; i < d.n_time * d.n_freq; ++i) out[i] = lost(random) ? 0u : 1u;
+    });
+}
+
+bf_status bf_shard_frequencies(const bf_shard* shard, float start_hz, float channel_width_hz,
+                               float* out) {
+    return guarded([&] {
+        const bf_shard& s = deref(shard);
+        validate_shard(s);
+        if (out == nullptr) invalid("out must not be null");
+        if (!std::isfinite(start_hz) || start_hz <= 0.0f || !std::isfinite(channel_width_hz) ||
+            channel_width_hz <= 0.0f)
+            invalid("frequency start and channel width must be positive");
+        // channel absolute_frequency_start + k of the full-band plan
+        // (channelized_frequencies over both shards, geometry.cpp:79-95)
+        for (size_t k = 0; k < s.local_frequency_count; ++k)
+            out[k] = start_hz + (float)(s.absolute_frequency_start + k) * channel_width_hz;
     });
 }
 

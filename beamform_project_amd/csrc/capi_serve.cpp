@@ -616,6 +616,14 @@ void create_server(const bf_server_config* c, std::unique_ptr<bf_server>* out) {
     s->cfg = *c;
     s->cfg.positions = nullptr;
     s->cfg.frequencies = nullptr;
+    s->cfg.shard = nullptr;
+    bf_shard shard{0, kFrequencyShardCount, kLocalFrequencyChannels, 0, 0, 1, 0, 1};
+    if (c->shard != nullptr) {
+        if (c->n_freq != kLocalFrequencyChannels)
+            invalid("a shard descriptor needs n_freq = 336 (one local shard)");
+        validate_shard(*c->shard);
+        shard = *c->shard;
+    }
     s->F = c->n_freq;
     s->A = c->n_ant;
     s->R = R;
@@ -663,8 +671,9 @@ void create_server(const bf_server_config* c, std::unique_ptr<bf_server>* out) {
         pos[3 * a + 1] = c->positions ? c->positions[3 * a + 1] : (float)(a / 8) * 0.6f;
         pos[3 * a + 2] = c->positions ? c->positions[3 * a + 2] : 0.0f;
     }
+    // channels shard.absolute_frequency_start + f of the full-band plan
     for (size_t f = 0; f < s->F; ++f) {
-        const float step = (float)f * 300000.0f;
+        const float step = (float)(shard.absolute_frequency_start + f) * 300000.0f;
         fr[f] = c->frequencies ? c->frequencies[f] : 300000000.0f + step;
     }
     hip_check(hipMalloc(reinterpret_cast<void**>(&s->d_pos), pos.size() * sizeof(float)),

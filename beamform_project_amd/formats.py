@@ -49,6 +49,10 @@ for _name, _args in {
     "bf_read_quantized_output_metadata": [_cp, _pm],
     "bf_write_packed_shard": [_cp, _cp, _cp, _psh, _vp, _sz, _vp, _sz, _pd],
     "bf_read_packed_shard": [_cp, _cp, _cp, _pd, _psh, _vp, _sz, _vp, _sz],
+    "bf_validate_packed_shard": [_psh, _vp, _sz, _vp, _sz, _pd],
+    "bf_validate_packed_shards": [_psh, C.POINTER(_vp), _sz, C.POINTER(_vp), _sz, _pd],
+    "bf_make_loss_mask": [_pd, C.c_uint32, C.c_float, _vp, _sz],
+    "bf_shard_frequencies": [_psh, C.c_float, C.c_float, _vp],
 }.items():
     _capi.register(_name, _args)
 
@@ -271,3 +275,43 @@ def read_packed_shard(payload_path, metadata_path, loss_mask_path, dims: Dimensi
          _path(loss_mask_path), C.byref(d), C.byref(s), payload.ctypes.data, payload.size,
          mask.ctypes.data, mask.size)
     return ShardDescriptor.from_c(s), payload, mask
+
+
+def validate_packed_shard(shard: ShardDescriptor, payload, loss_mask, dims: Dimensions) -> None:
+    """Reference src/synthetic_data.cpp:139-153."""
+    p = np.ascontiguousarray(payload, np.uint8).reshape(-1)
+    m = np.ascontiguousarray(loss_mask, np.uint8).reshape(-1)
+    s, d = shard.c(), dims.c()
+    call("bf_validate_packed_shard", C.byref(s), p.ctypes.data, p.size, m.ctypes.data, m.size,
+         C.byref(d))
+
+
+def validate_packed_shards(shards, payloads, loss_masks, dims: Dimensions) -> None:
+    """Reference src/synthetic_data.cpp:155-171: a two-shard set."""
+    ps = [np.ascontiguousarray(p, np.uint8).reshape(-1) for p in payloads]
+    ms = [np.ascontiguousarray(m, np.uint8).reshape(-1) for m in loss_masks]
+    if len(shards) != 2 or len(ps) != 2 or len(ms) != 2 or ps[0].size != ps[1].size \
+            or ms[0].size != ms[1].size:
+        raise ValueError("a shard set holds two shards of equal sizes")
+    sh = (Shard * 2)(shards[0].c(), shards[1].c())
+    pp = (C.c_void_p * 2)(ps[0].ctypes.data, ps[1].ctypes.data)
+    mm = (C.c_void_p * 2)(ms[0].ctypes.data, ms[1].ctypes.data)
+    d = dims.c()
+    call("bf_validate_packed_shards", sh, pp, ps[0].size, mm, ms[0].size, C.byref(d))
+
+
+def make_loss_mask(dims: Dimensions, seed: int, loss_probability: float) -> np.ndarray:
+    """Reference src/synthetic_data.cpp:121-137: [T][F] validity bytes."""
+    out = np.zeros((dims.n_time, dims.n_freq), np.uint8)
+    d = dims.c()
+    call("bf_make_loss_mask", C.byref(d), seed, loss_probability, out.ctypes.data, out.size)
+    return out
+
+
+def shard_frequencies(shard: ShardDescriptor, start_hz: float = 300e6,
+                      channel_width_hz: float = 300e3) -> np.ndarray:
+    """The shard's 336 channel frequencies of the full-band plan."""
+    out = np.zeros(shard.local_frequency_count, np.float32)
+    s = shard.c()
+    call("bf_shard_frequencies", C.byref(s), start_hz, channel_width_hz, out.ctypes.data)
+    return out

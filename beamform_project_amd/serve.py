@@ -24,7 +24,7 @@ class ServerConfigC(C.Structure):
     _fields_ = [("n_freq", _sz), ("n_ant", _sz), ("ring_spectra", _sz), ("max_window", _sz),
                 ("max_step_spectra", _sz), ("hbm_budget_bytes", _sz), ("max_requests", _sz),
                 ("positions", _vp), ("frequencies", _vp), ("kernel", C.c_int),
-                ("enable_graph", C.c_int), ("loss_masks", C.c_int)]
+                ("enable_graph", C.c_int), ("loss_masks", C.c_int), ("shard", _vp)]
 
 
 class RingSpanC(C.Structure):
@@ -117,6 +117,7 @@ class ServerConfig:
     kernel: Kernel = Kernel.SPLIT_F16
     enable_graph: bool = True
     loss_masks: bool = False                   # validity ring [ring][F]
+    shard: Optional[object] = None             # formats.ShardDescriptor of the stream
 
     def to_c(self, keep: list) -> ServerConfigC:
         pos = fr = None
@@ -126,11 +127,16 @@ class ServerConfig:
         if self.frequencies is not None:
             fr = np.ascontiguousarray(self.frequencies, np.float32)
             keep.append(fr)
+        sh = None
+        if self.shard is not None:
+            sh = self.shard.c()
+            keep.append(sh)
         return ServerConfigC(self.n_freq, self.n_ant, self.ring_spectra, self.max_window,
                              self.max_step_spectra, self.hbm_budget_bytes, self.max_requests,
                              None if pos is None else pos.ctypes.data,
                              None if fr is None else fr.ctypes.data, int(self.kernel),
-                             int(self.enable_graph), int(self.loss_masks))
+                             int(self.enable_graph), int(self.loss_masks),
+                             None if sh is None else C.addressof(sh))
 
 
 @dataclass

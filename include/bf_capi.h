@@ -608,6 +608,8 @@ typedef struct bf_server_config {
     int kernel;               /* SPLIT_F16, F16 or AUTO (= SPLIT_F16) */
     int enable_graph;
     int loss_masks;           /* 1: keep a validity ring and honour loss masks */
+    const bf_shard* shard;    /* the stream's shard (copied); with frequencies NULL its
+                                 channels of the reference plan; NULL: shard 0 */
 } bf_server_config;
 
 /* Ring rows of consecutive spectra handed to a producer: voltages
@@ -867,6 +869,24 @@ bf_status bf_read_packed_shard(const char* payload_path, const char* metadata_pa
                                bf_shard* shard_out, uint8_t* payload,
                                size_t payload_capacity, uint8_t* loss_mask,
                                size_t loss_mask_capacity);
+/* validate_packed_shard / validate_packed_shards (synthetic_data.cpp:139-171):
+ * a shard's descriptor, payload size and 0/1 mask; a two-shard set adds slot
+ * identity, distinct frequency origins and independent masks (shards[2],
+ * payloads[2], loss_masks[2]). */
+bf_status bf_validate_packed_shard(const bf_shard* shard, const uint8_t* payload,
+                                   size_t payload_bytes, const uint8_t* loss_mask,
+                                   size_t loss_mask_bytes, const bf_dims* dims);
+bf_status bf_validate_packed_shards(const bf_shard* shards, const uint8_t* const* payloads,
+                                    size_t payload_bytes, const uint8_t* const* loss_masks,
+                                    size_t loss_mask_bytes, const bf_dims* dims);
+/* make_loss_mask (synthetic_data.cpp:121-137): [T][F] bytes, 0 with
+ * probability loss_probability, from the same seeded generator. */
+bf_status bf_make_loss_mask(const bf_dims* dims, uint32_t seed, float loss_probability,
+                            uint8_t* out, size_t n);
+/* The shard's channel frequencies: channels absolute_frequency_start ..
+ * + 335 of the full-band plan start_hz + k * channel_width_hz. */
+bf_status bf_shard_frequencies(const bf_shard* shard, float start_hz, float channel_width_hz,
+                               float* out);
 
 #ifdef __cplusplus
 }

@@ -347,6 +347,7 @@ def ref_lib():
         "ref_read_packed_shard": [C.c_char_p, C.c_char_p, C.c_char_p, _sz, _sz, _sz, _sz,
                                   _u8p, _u8p, C.POINTER(C.c_uint64)],
         "ref_write_tiled_weights_file": [C.c_char_p, _f32p, _sz, _sz, _sz, _sz],
+        "ref_make_loss_mask": [_sz, _sz, _sz, C.c_uint32, C.c_float, _u8p],
         "ref_hex_targets": [_sz, C.c_float, C.c_float, _f32p, C.POINTER(_sz)],
         "ref_estimate_hex": [C.c_float, _sz, _sz, C.c_float, C.c_float, C.c_float,
                              C.POINTER(_sz)],

@@ -83,6 +83,14 @@ int ref_fft_beam_grid(std::size_t n_ant, std::size_t n_beams, float* out) {
     return guarded([&] { put_vec3s(fft_beam_grid(n_ant, n_beams), out); });
 }
 
+int ref_make_loss_mask(std::size_t T, std::size_t F, std::size_t A, std::uint32_t seed,
+                       float p, std::uint8_t* out) {
+    return guarded([&] {
+        const auto m = make_loss_mask(Dimensions{T, F, A, 1}, seed, p);
+        std::memcpy(out, m.data(), m.size());
+    });
+}
+
 int ref_hex_targets(std::size_t n, float u_max, float v_max, float* out, std::size_t* n_out) {
     return guarded([&] {
         const auto v = generate_hex_targets_cropped_fov(n, u_max, v_max);

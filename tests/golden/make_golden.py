@@ -15,6 +15,7 @@ Each case mirrors a case of the reference's own tests:
   tracker_moving_a32    tests/cuda/test_cuda_tracker_v2.cpp:45-58 (shortened)
   io_quantized_meta.txt src/io.cpp:149-183 (write_quantized_output_metadata)
   io_shard_*.bin/.txt   src/io.cpp:244-278 (write_packed_shard, T=2 A=32)
+  loss_mask_seed7       src/synthetic_data.cpp:121-137 (make_loss_mask, T=64)
   geometry_grids        tests/cpu/test_geometry.cpp (rectangular, FFT-bin, hex and
                         line grids, FFT bin selection, hex beam estimates)
   opt_tracker_*         tests/cpu/test_cpu_opt_beam_tracker.cpp:290-645 (the
@@ -223,6 +224,9 @@ def main() -> None:
     assert R.ref_default_beam_grid(7, 0.015, 0.05, d7) == 0
     grids["line_7_0.015_0.05"] = d7
     save("geometry_grids", **grids)
+    lm = np.zeros((64, F), np.uint8)
+    assert R.ref_make_loss_mask(64, F, 32, 7, 0.125, lm) == 0
+    save("loss_mask_seed7", seed=np.int64(7), probability=np.float32(0.125), mask=lm)
     # Adaptive DOA tracker (CpuOptBeamTracker::run_into) cases.
     for name, case in OPT_CASES.items():
         A, T, n = case["A"], case["T"], case.get("calls", 1)
