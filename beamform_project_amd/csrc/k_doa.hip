@@ -345,17 +345,34 @@ __global__ __launch_bounds__(64 * kBbParts) void broadband_kernel(
 
 constexpr int kMaxCandidates = 4096;
 constexpr int kPeaksLds = 4096;  // spectra up to this many angles staged in LDS
+constexpr int kRankMax = 1024;   // candidate lists up to this long are ranked in parallel
+
+// Angle of the parabola vertex through (q-1, q, q+1), the offset clamped to
+// +-0.5 cell (0 where the three points are not concave).
+__device__ __forceinline__ double refine_peak(const double* P, const double* theta, int q) {
+    const double pm = P[q - 1], pc = P[q], pp = P[q + 1];
+    const double den = pm - 2.0 * pc + pp;
+    double dlt = 0.0;
+    if (den < 0.0) {
+        dlt = 0.5 * (pm - pp) / den;
+        dlt = dlt > 0.5 ? 0.5 : (dlt < -0.5 ? -0.5 : dlt);
+    }
+    return theta[q] + dlt * (theta[q + 1] - theta[q]);
+}
 
 // Peaks of one broadband spectrum (workgroup 0: Capon, 1: Bartlett): local
 // maxima (P[q] > P[q-1] and P[q] >= P[q+1]) are found in parallel and
-// compacted in increasing q; one thread then refines each with the
-// parabola through (q-1, q, q+1) (vertex offset clamped to +-0.5 cell) and
-// keeps the max_peaks strongest, earlier angles first on ties.
+// compacted in increasing q, each is refined with the parabola through
+// (q-1, q, q+1), and the max_peaks strongest are kept, earlier angles first
+// on ties.  Up to kRankMax candidates, every candidate's slot is its rank in
+// that order, counted in parallel; longer lists go through one thread's
+// insertion into the output, which gives the same list.
 __global__ __launch_bounds__(kThreads) void peaks_kernel(
     const double* __restrict__ broadband, int n_theta, const double* __restrict__ theta,
     int max_peaks, double* __restrict__ peaks /* [2][max_peaks][2] theta, power */,
     int* __restrict__ n_peaks /* [2] */) {
     __shared__ int cand[kMaxCandidates];
+    __shared__ double cpow[kRankMax];
     __shared__ int wave_count[kThreads / 64];
     __shared__ int total;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -391,24 +408,42 @@ __global__ __launch_bounds__(kThreads) void peaks_kernel(
             for (int w = 0; w < kThreads / 64; ++w) total += wave_count[w];
         __syncthreads();
     }
-    if (tid != 0) return;
+    const int n_cand = total;  // uniform: read after the last barrier
     double* out = peaks + (size_t)blockIdx.x * max_peaks * 2;
+    if (n_cand <= kRankMax) {
+        for (int i = tid; i < n_cand; i += kThreads) cpow[i] = P[cand[i]];
+        const int n_out = n_cand < max_peaks ? n_cand : max_peaks;
+        for (int i = n_out + tid; i < max_peaks; i += kThreads) out[2 * i] = out[2 * i + 1] = 0.0;
+        __syncthreads();
+        // rank = candidates ahead in (power descending, q ascending); the
+        // ranks are a permutation of 0 .. n_cand-1, so every slot below
+        // n_out is written exactly once
+        for (int i = tid; i < n_cand; i += kThreads) {
+            const double pc = cpow[i];
+            int rank = 0;
+            for (int j = 0; j < n_cand; ++j) {
+                const double pj = cpow[j];
+                rank += (pj > pc || (pj == pc && j < i)) ? 1 : 0;
+            }
+            if (rank < max_peaks) {
+                out[2 * rank] = refine_peak(P, theta, cand[i]);
+                out[2 * rank + 1] = pc;
+            }
+        }
+        if (tid == 0) n_peaks[blockIdx.x] = n_out;
+        return;
+    }
+    if (tid != 0) return;
     for (int i = 0; i < max_peaks; ++i) out[2 * i] = out[2 * i + 1] = 0.0;
-    const bool listed = total <= kMaxCandidates;
-    const int n_scan = listed ? total : n_theta;
+    const bool listed = n_cand <= kMaxCandidates;
+    const int n_scan = listed ? n_cand : n_theta;
     int n = 0;
     for (int i = 0; i < n_scan; ++i) {
         const int q = listed ? cand[i] : i;
         if (!listed && !(q >= 1 && q + 1 < n_theta && P[q] > P[q - 1] && P[q] >= P[q + 1]))
             continue;
-        const double pm = P[q - 1], pc = P[q], pp = P[q + 1];
-        const double den = pm - 2.0 * pc + pp;
-        double dlt = 0.0;
-        if (den < 0.0) {
-            dlt = 0.5 * (pm - pp) / den;
-            dlt = dlt > 0.5 ? 0.5 : (dlt < -0.5 ? -0.5 : dlt);
-        }
-        const double th = theta[q] + dlt * (theta[q + 1] - theta[q]);
+        const double pc = P[q];
+        const double th = refine_peak(P, theta, q);
         // insertion into the top-max_peaks list by power
         int pos = n < max_peaks ? n : max_peaks;
         while (pos > 0 && out[2 * (pos - 1) + 1] < pc) {

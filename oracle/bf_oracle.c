@@ -835,6 +835,29 @@ double or_quadratic_peak_offset(double pm, double pc, double pp) {
     return d;
 }
 
+size_t or_doa_peaks(const double* P, const double* theta, size_t n, size_t max_peaks,
+                    double* out) {
+    size_t count = 0;
+    for (size_t i = 0; i < 2 * max_peaks; ++i) out[i] = 0.0;
+    for (size_t q = 1; q + 1 < n; ++q) {
+        if (!(P[q] > P[q - 1] && P[q] >= P[q + 1])) continue;
+        /* slot: after every kept peak at least as strong (earlier q wins ties) */
+        size_t slot = 0;
+        while (slot < count && out[2 * slot + 1] >= P[q]) ++slot;
+        if (slot >= max_peaks) continue;
+        const size_t last = count < max_peaks ? count : max_peaks - 1;
+        for (size_t s = last; s > slot; --s) {
+            out[2 * s] = out[2 * (s - 1)];
+            out[2 * s + 1] = out[2 * (s - 1) + 1];
+        }
+        const double d = or_quadratic_peak_offset(P[q - 1], P[q], P[q + 1]);
+        out[2 * slot] = theta[q] + d * (theta[q + 1] - theta[q]);
+        out[2 * slot + 1] = P[q];
+        if (count < max_peaks) ++count;
+    }
+    return count;
+}
+
 int or_doa_update(const uint8_t* x, size_t T, size_t F, size_t A, size_t t0, size_t K,
                   size_t M, double alpha, double load, int first, const float* freqs,
                   const float* xpos, size_t n_theta, const double* theta,

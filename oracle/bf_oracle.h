@@ -159,6 +159,14 @@ double or_bartlett_f64(const double* R, const double* a, size_t M);
  * textbook vertex of the parabola through the three samples. */
 double or_quadratic_peak_offset(double p_minus, double p_centre, double p_plus);
 
+/* Peak list of one broadband spectrum P[0..n): the local maxima
+ * (P[q] > P[q-1] and P[q] >= P[q+1], 0 < q < n-1), each moved to the vertex
+ * of its parabola (or_quadratic_peak_offset times theta[q+1] - theta[q]),
+ * the max_peaks strongest, strongest first and the earlier angle first on
+ * equal power.  out[2i] = angle, out[2i+1] = power; returns the count. */
+size_t or_doa_peaks(const double* P, const double* theta, size_t n, size_t max_peaks,
+                    double* out);
+
 /* Full C5 update over all channels: covariance -> smoothing -> /(K*L) ->
  * diagonal load -> fold with alpha (first update copies) -> Cholesky ->
  * Capon and Bartlett scan over n_theta angles of a ULA with element x

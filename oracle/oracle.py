@@ -85,6 +85,7 @@ def _load():
         "or_capon_from_factor_f64": (C.c_double, [_f64p, _f64p, _sz]),
         "or_bartlett_f64": (C.c_double, [_f64p, _f64p, _sz]),
         "or_quadratic_peak_offset": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+        "or_doa_peaks": (_sz, [_f64p, _f64p, _sz, _sz, _f64p]),
         "or_doa_update": (C.c_int, [_u8p, _sz, _sz, _sz, _sz, _sz, _sz, C.c_double, C.c_double,
                                     C.c_int, _f32p, _f32p, _sz, _f64p, _f64p, _f64p, _f64p,
                                     _i32p]),
@@ -310,6 +311,14 @@ def doa_update(x, T, F, A, t0, K, M, alpha, load, first, freqs, xpos, theta, R_s
                              np.ascontiguousarray(theta, np.float64), R_state, capon, bart, ok),
            "doa_update")
     return capon, bart, ok
+
+
+def doa_peaks(P, theta, max_peaks):
+    """[count][2] (angle, power) peak list of one broadband spectrum."""
+    P = np.ascontiguousarray(P, np.float64)
+    out = np.zeros((max(max_peaks, 1), 2), np.float64)
+    n = LIB.or_doa_peaks(P, np.ascontiguousarray(theta, np.float64), len(P), max_peaks, out)
+    return out[:n]
 
 
 # ------------------------------------------------------- reference (_ref)

@@ -111,6 +111,27 @@ def test_single_source_recovery():
         assert abs(top - 17.3) < 0.05, (key, top)
 
 
+@pytest.mark.parametrize("n_theta,max_peaks,sources", [
+    (721, 4, [-20.0, 11.5]), (721, 64, [3.0]), (2001, 1, [-44.0, 30.0]), (2001, 64, []),
+    (91, 64, [])])
+def test_peak_lists_match_oracle(n_theta, max_peaks, sources):
+    # the device peak list of each broadband spectrum equals the oracle's on
+    # the same spectrum: count, order (power descending, earlier angle on
+    # ties) and powers exactly, vertex angles to 1e-12 rad
+    A, M, K = 64, 32, 256
+    x = ula_sources(K, A, sources, amp=2.5, noise=1.5, seed=11)
+    est = DoaEstimator(DoaConfig(FR, n_ant=A, snapshots=K, subarray=M, n_theta=n_theta,
+                                 max_peaks=max_peaks))
+    est.update(api.DeviceArray.from_numpy(x), K, 0)
+    r = est.results()
+    for key in ("capon", "bartlett"):
+        want = O.doa_peaks(r[key], est.theta, max_peaks)
+        got = r[key + "_peaks"]
+        assert got.shape == want.shape, key
+        assert np.array_equal(got[:, 1], want[:, 1]), key
+        assert np.max(np.abs(got[:, 0] - want[:, 0]), initial=0.0) < 1e-12, key
+
+
 def test_coherent_sources_resolved_with_smoothing():
     A, M, K = 64, 48, 1024
     truth = [-30.0, 0.0, 25.0]

@@ -320,6 +320,35 @@ def test_quadratic_peak_offset_closed_form():
     assert O.LIB.or_quadratic_peak_offset(q(-1), q(0), q(1)) == 0.5
 
 
+def test_doa_peaks_against_find_peaks_and_closed_forms():
+    from scipy.signal import find_peaks
+    # the local maxima of tie-free data are scipy's peaks; the list holds
+    # the strongest, in descending power
+    rng = np.random.default_rng(5)
+    theta = np.linspace(-1.5, 1.5, 721)
+    for max_peaks in (1, 4, 1000):
+        P = rng.random(721)
+        idx, _ = find_peaks(P)
+        want = idx[np.argsort(-P[idx], kind="stable")][:max_peaks]
+        got = O.doa_peaks(P, theta, max_peaks)
+        assert len(got) == len(want)
+        assert np.array_equal(got[:, 1], P[want])
+        assert np.all(np.abs(got[:, 0] - theta[want]) <= 0.5 * (theta[1] - theta[0]) + 1e-15)
+    # exact parabolas: the vertex comes back, whatever the grid offset
+    th = np.linspace(-1.0, 1.0, 201)
+    vert = [(-0.4031, 3.0), (0.1207, 5.0), (0.6655, 4.0)]
+    P = np.max([h - 50.0 * (th - c) ** 2 for c, h in vert], axis=0)
+    got = O.doa_peaks(P, th, 3)
+    assert np.allclose(got[:, 0], [0.1207, 0.6655, -0.4031], atol=1e-12)
+    # equal powers: the earlier angle first; a plateau counts at its left edge;
+    # no peak at either end; fewer peaks than slots
+    P = np.array([9.0, 0.0, 2.0, 0.0, 3.0, 3.0, 0.0, 2.0, 0.0, 9.0])
+    got = O.doa_peaks(P, np.arange(10.0), 8)
+    assert np.array_equal(got[:, 1], [3.0, 2.0, 2.0])
+    assert got[1, 0] < got[2, 0] and got[0, 0] <= 4.5
+    assert len(O.doa_peaks(np.zeros(5), np.arange(5.0), 3)) == 0
+
+
 def test_doa_update_recovers_single_source():
     A, M, K = 64, 48, 256
     fr = np.linspace(400e6, 800e6, F).astype(np.float32)
