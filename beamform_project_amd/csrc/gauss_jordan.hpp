@@ -1,9 +1,10 @@
 // gauss_jordan.hpp -- in-place inverse of a Hermitian positive-definite
-// complex FP64 matrix (M <= 64) by Gauss-Jordan elimination without
-// pivoting, for one workgroup of (64 / BS)^2 threads.  Thread t holds the
-// BS x BS block of rows BS (t / (64 / BS)) .. and columns BS (t % (64 / BS))
-// .. in registers (identity outside M x M); BS = 2 (1024 threads) keeps each
-// wave's chain per pivot short.  Per pivot k the owners publish row k and
+// complex FP64 matrix (M <= NB BS <= 64) by Gauss-Jordan elimination
+// without pivoting, for one workgroup of NB^2 threads.  Thread t holds the
+// BS x BS block of rows BS (t / NB) .. and columns BS (t % NB) .. in
+// registers (identity outside M x M); NB BS just above M keeps the padding,
+// which costs the same arithmetic as the matrix, small (NB = 16: BS = 3 for
+// M <= 48, BS = 4 for M <= 64).  Per pivot k the owners publish row k and
 // column k to LDS (double-buffered, so one barrier per pivot) and every
 // thread updates its block without branches.  Without pivoting on a Hermitian positive-definite
 // matrix the k-th pivot equals the Cholesky pivot L_kk^2, so a factorisation
@@ -22,14 +23,15 @@ struct GjShared {
 
 // Loads the block of a row-major M x M matrix (re, im at 2 (i M + j)) held
 // in `src` (global or LDS) into registers.
-template <int BS>
-__device__ __forceinline__ int gj_row0(int tid) { return BS * (tid / (64 / BS)); }
-template <int BS>
-__device__ __forceinline__ int gj_col0(int tid) { return BS * (tid % (64 / BS)); }
+template <int BS, int NB = 64 / BS>
+__device__ __forceinline__ int gj_row0(int tid) { return BS * (tid / NB); }
+template <int BS, int NB = 64 / BS>
+__device__ __forceinline__ int gj_col0(int tid) { return BS * (tid % NB); }
 
-template <int BS, typename Cd>
+template <int BS, int NB = 64 / BS, typename Cd>
 __device__ __forceinline__ void gj_load(Cd (&a)[BS][BS], const double* src, int M, int tid) {
-    const int i0 = gj_row0<BS>(tid), j0 = gj_col0<BS>(tid);
+    static_assert(NB * BS <= 64, "the published row and column hold 64 entries");
+    const int i0 = gj_row0<BS, NB>(tid), j0 = gj_col0<BS, NB>(tid);
 #pragma unroll
     for (int u = 0; u < BS; ++u)
 #pragma unroll
@@ -42,11 +44,13 @@ __device__ __forceinline__ void gj_load(Cd (&a)[BS][BS], const double* src, int 
         }
 }
 
-// Inverts in place; returns false (uniformly) at the first pivot k < M that
-// is not above `floor`, leaving the block unspecified.
-template <int BS, typename Cd>
+// Inverts in place (M <= NB BS, which the caller guarantees); returns false
+// (uniformly) at the first pivot k < M that is not above `floor`, leaving
+// the block unspecified.
+template <int BS, int NB = 64 / BS, typename Cd>
 __device__ bool gj_invert(Cd (&a)[BS][BS], int M, double floor, GjShared<Cd>& sh, int tid) {
-    const int i0 = gj_row0<BS>(tid), j0 = gj_col0<BS>(tid);
+    static_assert(NB * BS <= 64, "the published row and column hold 64 entries");
+    const int i0 = gj_row0<BS, NB>(tid), j0 = gj_col0<BS, NB>(tid);
     // pivots beyond M act on the identity padding only: skipped
     for (int k = 0; k < M; ++k) {
         const int buf = k & 1;

@@ -202,8 +202,10 @@ __device__ __forceinline__ cd cmul(cd a, cd b) {
 // One workgroup of 256 threads per channel (4 x 4 Gauss-Jordan blocks; 2 x 2
 // blocks on 1024 threads measured slower: 16-wave barriers, twice the LDS reads).
 // buf0/buf1: M x M complex double in LDS.
-constexpr int kGjBlock = 4;
-constexpr int kFoldThreads = (64 / kGjBlock) * (64 / kGjBlock);  // 256
+constexpr int kFoldSide = 16;                          // Gauss-Jordan blocks per side
+constexpr int kFoldThreads = kFoldSide * kFoldSide;   // 256
+// BS x BS Gauss-Jordan blocks: 3 for M <= 48, 4 for M <= 64.
+template <int BS>
 __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
     const int* __restrict__ rs, double* __restrict__ state /* [F][M][M][2] */, int first,
     int M, int K, int L, double alpha, double load, const float* __restrict__ freqs,
@@ -263,17 +265,17 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
     // the Cholesky pivots, so R fails the factorisation exactly when a
     // pivot is not positive.
     {
-        cd blk[kGjBlock][kGjBlock];
-        gj_load<kGjBlock>(blk, reinterpret_cast<const double*>(buf0), M, tid);
+        cd blk[BS][BS];
+        gj_load<BS, kFoldSide>(blk, reinterpret_cast<const double*>(buf0), M, tid);
         __syncthreads();  // every block read before Q overwrites buf0
-        const bool ok_gj = gj_invert<kGjBlock>(blk, M, 0.0, gj, tid);
+        const bool ok_gj = gj_invert<BS, kFoldSide>(blk, M, 0.0, gj, tid);
         if (tid == 0) s_ok = ok_gj ? 1 : 0;
         if (ok_gj) {
-            const int i0 = gj_row0<kGjBlock>(tid), j0 = gj_col0<kGjBlock>(tid);
+            const int i0 = gj_row0<BS, kFoldSide>(tid), j0 = gj_col0<BS, kFoldSide>(tid);
 #pragma unroll
-            for (int u = 0; u < kGjBlock; ++u)
+            for (int u = 0; u < BS; ++u)
 #pragma unroll
-                for (int v = 0; v < kGjBlock; ++v)
+                for (int v = 0; v < BS; ++v)
                     if (i0 + u < M && j0 + v < M) buf0[(i0 + u) * M + j0 + v] = blk[u][v];
         }
         __syncthreads();
@@ -484,13 +486,20 @@ void launch_doa_fold_scan(hipStream_t stream, const int* rs, double* state, bool
                           const float* freqs, double x_step, size_t n_theta,
                           const double* sin_theta, double* capon, double* bartlett,
                           int* chol_ok) {
+    if (M == 0 || M > 4 * kFoldSide) invalid("the DOA fold supports 1 <= subarray <= 64");
     const size_t smem = (2 * M * M + 2 * M) * sizeof(double) * 2;
-    hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(fold_scan_kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
-              "hipFuncSetAttribute");
-    hipLaunchKernelGGL(fold_scan_kernel, dim3((unsigned)F), dim3(kFoldThreads), smem, stream, rs,
-                       state, first ? 1 : 0, (int)M, (int)K, (int)L, alpha, load, freqs, x_step,
-                       (int)n_theta, sin_theta, capon, bartlett, chol_ok);
+    auto go = [&](auto kernel) {
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
+                  "hipFuncSetAttribute");
+        hipLaunchKernelGGL(kernel, dim3((unsigned)F), dim3(kFoldThreads), smem, stream, rs, state,
+                           first ? 1 : 0, (int)M, (int)K, (int)L, alpha, load, freqs, x_step,
+                           (int)n_theta, sin_theta, capon, bartlett, chol_ok);
+    };
+    if (M <= 3 * kFoldSide)
+        go(fold_scan_kernel<3>);
+    else
+        go(fold_scan_kernel<4>);
     hip_check(hipGetLastError(), "fold_scan_kernel launch");
 }
 
