@@ -21,15 +21,16 @@ struct GjShared {
     Cd col[2][64];
 };
 
-// Loads the block of a row-major M x M matrix (re, im at 2 (i M + j)) held
-// in `src` (global or LDS) into registers.
+// Loads the block of a row-major M x M matrix with row stride ld (re, im at
+// 2 (i ld + j)) held in `src` (global or LDS) into registers.
 template <int BS, int NB = 64 / BS>
 __device__ __forceinline__ int gj_row0(int tid) { return BS * (tid / NB); }
 template <int BS, int NB = 64 / BS>
 __device__ __forceinline__ int gj_col0(int tid) { return BS * (tid % NB); }
 
 template <int BS, int NB = 64 / BS, typename Cd>
-__device__ __forceinline__ void gj_load(Cd (&a)[BS][BS], const double* src, int M, int tid) {
+__device__ __forceinline__ void gj_load(Cd (&a)[BS][BS], const double* src, int M, int ld,
+                                        int tid) {
     static_assert(NB * BS <= 64, "the published row and column hold 64 entries");
     const int i0 = gj_row0<BS, NB>(tid), j0 = gj_col0<BS, NB>(tid);
 #pragma unroll
@@ -38,7 +39,7 @@ __device__ __forceinline__ void gj_load(Cd (&a)[BS][BS], const double* src, int 
         for (int v = 0; v < BS; ++v) {
             const int i = i0 + u, j = j0 + v;
             if (i < M && j < M)
-                a[u][v] = Cd{src[2 * (i * M + j)], src[2 * (i * M + j) + 1]};
+                a[u][v] = Cd{src[2 * (i * ld + j)], src[2 * (i * ld + j) + 1]};
             else
                 a[u][v] = Cd{i == j ? 1.0 : 0.0, 0.0};
         }

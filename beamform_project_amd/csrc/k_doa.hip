@@ -89,35 +89,36 @@ __global__ __launch_bounds__(kThreads) void covariance_kernel(
     const size_t row_stride = (size_t)F * A;
     const uint8_t* xf = x + (size_t)f * A;
     // Staging work: 64 snapshots x A antennas = 16 snapshot quads x A/4
-    // antenna quads, QPT per thread.  The dwords of the next 64-snapshot
-    // chunk are loaded into registers while this chunk's MFMAs run.
+    // antenna quads, QPT per thread.  A ring of kDepth chunks is in flight
+    // in registers: chunk c is staged while chunks c+1 .. c+kDepth-1 load.
+    // Rows past the window read row t0 (in bounds: the launcher checks
+    // t0 + K <= T), so the loads need no branches; their values are replaced
+    // by zeros when the chunk is consumed (a select at load time would wait
+    // for the load).
     constexpr int QUADS = 16 * (A / 4);
     constexpr int QPT = (QUADS + kThreads - 1) / kThreads;
-    unsigned nxt[QPT][4];
-    auto load_chunk = [&](int k0) {
+    constexpr int kDepth = 4;
+    unsigned ring[kDepth][QPT][4];
+    auto in_window = [&](int j, int i, int k0) {
+        const int q = tid + j * kThreads;
+        return q < QUADS && k0 + 4 * (q / (A / 4)) + i < K;
+    };
+    auto load_chunk = [&](unsigned (&dst)[QPT][4], int k0) {
 #pragma unroll
         for (int j = 0; j < QPT; ++j) {
             const int q = tid + j * kThreads;
             const int k4 = q / (A / 4);
-            const int a4 = q - k4 * (A / 4);
+            const int a4 = q < QUADS ? q - k4 * (A / 4) : 0;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int kk = k0 + 4 * k4 + i;
-                nxt[j][i] = 0u;
-                if (q < QUADS && kk < K && t0 + kk < T)
-                    nxt[j][i] = *reinterpret_cast<const unsigned*>(
-                        xf + (size_t)(t0 + kk) * row_stride + 4 * a4);
+                const int kk = in_window(j, i, k0) ? k0 + 4 * k4 + i : 0;
+                dst[j][i] = *reinterpret_cast<const unsigned*>(
+                    xf + (size_t)(t0 + kk) * row_stride + 4 * a4);
             }
         }
     };
-    load_chunk(0);
-    for (int k0 = 0; k0 < K; k0 += 64) {
-        unsigned cur[QPT][4];
-#pragma unroll
-        for (int j = 0; j < QPT; ++j)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) cur[j][i] = nxt[j][i];
-        if (k0 + 64 < K) load_chunk(k0 + 64);
+    // one chunk: transpose into LDS, then the MFMA step of every tile
+    auto process = [&](const unsigned (&cur)[QPT][4]) {
 #pragma unroll
         for (int jq = 0; jq < QPT; ++jq) {
             const int q = tid + jq * kThreads;
@@ -127,11 +128,17 @@ __global__ __launch_bounds__(kThreads) void covariance_kernel(
             const unsigned* d = cur[jq];
             unsigned e[4];
             transpose4x4(d, e);
+            // Lanes of one instruction share k4 in groups of 16 consecutive
+            // a4, whose rows lie 8 apart (a multiple of 32 banks at this
+            // stride); writing antenna (jj + a4) mod 4 in instruction jj
+            // spreads a group over 4 bank groups.
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int jj = 0; jj < 4; ++jj) {
+                const int j = (jj + a4) & 3;
+                const unsigned ej = j == 0 ? e[0] : j == 1 ? e[1] : j == 2 ? e[2] : e[3];
                 const int a = 4 * a4 + j;
-                const unsigned re = (e[j] << 4) & 0xF0F0F0F0u;  // 16 * real
-                const unsigned im = e[j] & 0xF0F0F0F0u;         // 16 * imag
+                const unsigned re = (ej << 4) & 0xF0F0F0F0u;  // 16 * real
+                const unsigned im = ej & 0xF0F0F0F0u;         // 16 * imag
                 *reinterpret_cast<unsigned*>(tr + (2 * a) * kTStride + 4 * k4) = re;
                 *reinterpret_cast<unsigned*>(tr + (2 * a + 1) * kTStride + 4 * k4) = im;
             }
@@ -152,6 +159,36 @@ __global__ __launch_bounds__(kThreads) void covariance_kernel(
             }
         }
         __syncthreads();
+    };
+    auto take = [&](int s_k0, const unsigned (&slot)[QPT][4], unsigned (&cur)[QPT][4]) {
+#pragma unroll
+        for (int j = 0; j < QPT; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cur[j][i] = in_window(j, i, s_k0) ? slot[j][i] : 0u;
+    };
+#pragma unroll
+    for (int s = 0; s < kDepth; ++s) load_chunk(ring[s], 64 * s);
+    // full groups of kDepth chunks: each slot is consumed, then refilled
+    // unconditionally (past the window it reads row t0), so every slot has
+    // one producer and the ring needs no register copies
+    int kr = 0;
+    for (; kr + 64 * kDepth <= K; kr += 64 * kDepth) {
+#pragma unroll
+        for (int s = 0; s < kDepth; ++s) {
+            unsigned cur[QPT][4];
+            take(kr + 64 * s, ring[s], cur);
+            load_chunk(ring[s], kr + 64 * (s + kDepth));
+            process(cur);
+        }
+    }
+    // the last K mod (64 kDepth) snapshots: up to kDepth chunks, already loaded
+#pragma unroll
+    for (int s = 0; s < kDepth; ++s) {
+        if (kr + 64 * s < K) {  // uniform
+            unsigned cur[QPT][4];
+            take(kr + 64 * s, ring[s], cur);
+            process(cur);
+        }
     }
     // G tiles -> LDS: lane holds column l%16, rows 4*(l/16) + r.
 #pragma unroll
@@ -175,19 +212,50 @@ __global__ __launch_bounds__(kThreads) void covariance_kernel(
             r_full[(((size_t)f * A + r) * A + c) * 2 + 1] = (r <= c ? gim : -gim) / 256;
         }
     }
+    // Rs[a][b] = sum_{l<L} G'(a+l, b+l) on the diagonals b = a + d >= 0,
+    // with G'(r, c) = (G_rr + G_ii, G_ir - G_ri) / 256 (every G entry is a
+    // multiple of 256).  Each thread walks a segment of kSeg entries of one
+    // diagonal as a running sum, Rs(a+1, b+1) = Rs(a, b) - G'(a, b) +
+    // G'(a+L, b+L), and writes the entry and its conjugate (b, a): integers,
+    // so the order of the sums does not matter.
+    constexpr int kSeg = 8;
     const int L = A - M + 1;
-    for (int e = tid; e < M * M; e += kThreads) {
-        const int r = e / M, c = e - (e / M) * M;
-        const int a = r <= c ? r : c, b = r <= c ? c : r;  // upper entry
+    auto gp = [&](int r, int c, int& re, int& im) {  // r <= c
+        re = (g[(2 * r) * (C + 1) + 2 * c] + g[(2 * r + 1) * (C + 1) + 2 * c + 1]) / 256;
+        im = (g[(2 * r + 1) * (C + 1) + 2 * c] - g[(2 * r) * (C + 1) + 2 * c + 1]) / 256;
+    };
+    int2* rs2 = reinterpret_cast<int2*>(rs_out) + (size_t)f * M * M;
+    for (int w = tid;; w += kThreads) {
+        int d = 0, seg = w;
+        while (d < M) {
+            const int ns = (M - d + kSeg - 1) / kSeg;
+            if (seg < ns) break;
+            seg -= ns;
+            ++d;
+        }
+        if (d >= M) break;
+        const int a0 = seg * kSeg;
+        const int a1 = min(a0 + kSeg, M - d);
         int sre = 0, sim = 0;
         for (int l = 0; l < L; ++l) {
-            const int rr = l + a, cc = l + b;
-            // every G entry is a multiple of 256 (16 x 16 scaling)
-            sre += (g[(2 * rr) * (C + 1) + 2 * cc] + g[(2 * rr + 1) * (C + 1) + 2 * cc + 1]) / 256;
-            sim += (g[(2 * rr + 1) * (C + 1) + 2 * cc] - g[(2 * rr) * (C + 1) + 2 * cc + 1]) / 256;
+            int re, im;
+            gp(a0 + l, a0 + d + l, re, im);
+            sre += re;
+            sim += im;
         }
-        rs_out[(((size_t)f * M + r) * M + c) * 2] = sre;
-        rs_out[(((size_t)f * M + r) * M + c) * 2 + 1] = r <= c ? sim : -sim;
+        for (int a = a0; a < a1; ++a) {
+            if (a > a0) {
+                int re, im;
+                gp(a - 1, a - 1 + d, re, im);
+                sre -= re;
+                sim -= im;
+                gp(a + L - 1, a + L - 1 + d, re, im);
+                sre += re;
+                sim += im;
+            }
+            rs2[a * M + a + d] = make_int2(sre, sim);
+            if (d > 0) rs2[(a + d) * M + a] = make_int2(sre, -sim);
+        }
     }
 }
 
@@ -199,9 +267,12 @@ __device__ __forceinline__ cd cmul(cd a, cd b) {
     return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
-// One workgroup of 256 threads per channel (4 x 4 Gauss-Jordan blocks; 2 x 2
-// blocks on 1024 threads measured slower: 16-wave barriers, twice the LDS reads).
-// buf0/buf1: M x M complex double in LDS.
+// One workgroup of 256 threads (16 x 16 Gauss-Jordan blocks) per channel;
+// 2 x 2 blocks on 1024 threads measured slower (16-wave barriers, twice
+// the LDS reads).
+// buf0: M x M complex double in LDS, row stride M + 1 (a stride of M
+// complex doubles is a multiple of the 32 banks, which would put every
+// lane of the diagonal walks below on one bank).
 constexpr int kFoldSide = 16;                          // Gauss-Jordan blocks per side
 constexpr int kFoldThreads = kFoldSide * kFoldSide;   // 256
 // BS x BS Gauss-Jordan blocks: 3 for M <= 48, 4 for M <= 64.
@@ -212,9 +283,9 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
     double x_step, int n_theta, const double* __restrict__ sin_theta,
     double* __restrict__ capon, double* __restrict__ bartlett, int* __restrict__ chol_ok) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int MP = M + 1;
     cd* buf0 = reinterpret_cast<cd*>(smem);
-    cd* buf1 = buf0 + M * M;
-    cd* qb = buf1 + M * M;      // [M] Bartlett diagonal sums, d = 0..M-1
+    cd* qb = buf0 + M * MP;     // [M] Bartlett diagonal sums, d = 0..M-1
     cd* qc = qb + M;            // [M] Capon diagonal sums
     __shared__ double s_eps;
     __shared__ int s_ok;
@@ -226,12 +297,12 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
     // R_block = Rs / (K L)
     for (int e = tid; e < MM; e += kFoldThreads) {
         const int* p = rs + ((size_t)f * MM + e) * 2;
-        buf0[e] = cd{(double)p[0] * scale, (double)p[1] * scale};
+        buf0[(e / M) * MP + e % M] = cd{(double)p[0] * scale, (double)p[1] * scale};
     }
     __syncthreads();
     if (tid == 0) {
         double tr = 0.0;
-        for (int i = 0; i < M; ++i) tr += buf0[i * M + i].re;
+        for (int i = 0; i < M; ++i) tr += buf0[i * MP + i].re;
         s_eps = load > 0.0 ? load * tr / (double)M : 0.0;
         s_ok = 1;
     }
@@ -239,7 +310,8 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
     // diagonal load, then fold into the carried state
     double* st = state + (size_t)f * MM * 2;
     for (int e = tid; e < MM; e += kFoldThreads) {
-        cd v = buf0[e];
+        const int le = (e / M) * MP + e % M;
+        cd v = buf0[le];
         if (e / M == e % M) v.re += s_eps;
         if (!first) {
             v.re = alpha * st[2 * e] + (1.0 - alpha) * v.re;
@@ -247,14 +319,14 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
         }
         st[2 * e] = v.re;
         st[2 * e + 1] = v.im;
-        buf0[e] = v;
+        buf0[le] = v;
     }
     __syncthreads();
     // Bartlett diagonal sums of R: qb[d] = sum_c R[c+d][c]
     for (int d = tid; d < M; d += kFoldThreads) {
         cd s{0.0, 0.0};
         for (int c = 0; c + d < M; ++c) {
-            const cd v = buf0[(c + d) * M + c];
+            const cd v = buf0[(c + d) * MP + c];
             s.re += v.re;
             s.im += v.im;
         }
@@ -266,7 +338,7 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
     // pivot is not positive.
     {
         cd blk[BS][BS];
-        gj_load<BS, kFoldSide>(blk, reinterpret_cast<const double*>(buf0), M, tid);
+        gj_load<BS, kFoldSide>(blk, reinterpret_cast<const double*>(buf0), M, MP, tid);
         __syncthreads();  // every block read before Q overwrites buf0
         const bool ok_gj = gj_invert<BS, kFoldSide>(blk, M, 0.0, gj, tid);
         if (tid == 0) s_ok = ok_gj ? 1 : 0;
@@ -276,7 +348,7 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
             for (int u = 0; u < BS; ++u)
 #pragma unroll
                 for (int v = 0; v < BS; ++v)
-                    if (i0 + u < M && j0 + v < M) buf0[(i0 + u) * M + j0 + v] = blk[u][v];
+                    if (i0 + u < M && j0 + v < M) buf0[(i0 + u) * MP + j0 + v] = blk[u][v];
         }
         __syncthreads();
     }
@@ -285,7 +357,7 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
     for (int d = tid; d < M; d += kFoldThreads) {
         cd s{0.0, 0.0};
         for (int c = 0; c + d < M; ++c) {
-            const cd v = buf0[(c + d) * M + c];
+            const cd v = buf0[(c + d) * MP + c];
             s.re += v.re;
             s.im += v.im;
         }
@@ -468,6 +540,7 @@ __global__ __launch_bounds__(kThreads) void peaks_kernel(
 
 void launch_doa_covariance(hipStream_t stream, const uint8_t* x, size_t T, size_t F, size_t A,
                            size_t t0, size_t K, size_t M, int* rs, int* r_full) {
+    if (K == 0 || t0 + K > T) invalid("the covariance window exceeds the time axis");
     const unsigned grid = (unsigned)(8 * ceil_div(F, 8));
 #define BF_COV(A_)                                                                            \
     hipLaunchKernelGGL(covariance_kernel<A_>, dim3(grid), dim3(kThreads), 0, stream, x,        \
@@ -487,7 +560,7 @@ void launch_doa_fold_scan(hipStream_t stream, const int* rs, double* state, bool
                           const double* sin_theta, double* capon, double* bartlett,
                           int* chol_ok) {
     if (M == 0 || M > 4 * kFoldSide) invalid("the DOA fold supports 1 <= subarray <= 64");
-    const size_t smem = (2 * M * M + 2 * M) * sizeof(double) * 2;
+    const size_t smem = (M * (M + 1) + 2 * M) * sizeof(double) * 2;
     auto go = [&](auto kernel) {
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),

@@ -257,7 +257,7 @@ __global__ __launch_bounds__(kFactorThreads) void opt_factor_kernel(
     const int MM = M * M;
     const double* R = state + (size_t)f * MM * 2;
     cd a[kGjBlock][kGjBlock];
-    gj_load<kGjBlock>(a, R, M, tid);
+    gj_load<kGjBlock>(a, R, M, M, tid);
     const bool ok = gj_invert<kGjBlock>(a, M, 1e-12, sh, tid);
     if (!ok) {
         float2* Q = scanmat + (size_t)f * MM;
