@@ -35,7 +35,10 @@ def ula_sources(K, A, thetas_deg, amp=3.0, noise=1.0, seed=0, coherent=True, T=N
     return ((re & 0xF) | ((im & 0xF) << 4)).astype(np.uint8)
 
 
-@pytest.mark.parametrize("A,M,K", [(64, 64, 1024), (64, 48, 1024), (32, 20, 300), (64, 1, 64)])
+# K covers whole groups of four 64-snapshot chunks (1024), a tail of four
+# chunks with a partial last one (255), and tails of one to three chunks
+@pytest.mark.parametrize("A,M,K", [(64, 64, 1024), (64, 48, 1024), (32, 20, 300), (64, 1, 64),
+                                   (64, 40, 255), (32, 32, 193), (64, 48, 320)])
 def test_covariance_bit_exact(A, M, K):
     T = K + 37
     x = O.make_uniform_int4(T * F * A, A + M).reshape(T, F, A)
