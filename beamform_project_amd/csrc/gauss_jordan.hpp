@@ -5,10 +5,12 @@
 // registers (identity outside M x M); NB BS just above M keeps the padding,
 // which costs the same arithmetic as the matrix, small (NB = 16: BS = 3 for
 // M <= 48, BS = 4 for M <= 64).  Per pivot k the owners publish row k and
-// column k to LDS (double-buffered, so one barrier per pivot) and every
-// thread updates its block without branches.  Without pivoting on a Hermitian positive-definite
-// matrix the k-th pivot equals the Cholesky pivot L_kk^2, so a factorisation
-// test on it carries over unchanged.
+// column k to LDS (double-buffered, so one barrier per pivot), every thread
+// applies the elimination update to its whole block, and the threads
+// holding row or column k then overwrite those entries.  Without pivoting
+// on a Hermitian positive-definite matrix the k-th pivot equals the
+// Cholesky pivot L_kk^2, so a factorisation test on it carries over
+// unchanged.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -72,26 +74,42 @@ __device__ bool gj_invert(Cd (&a)[BS][BS], int M, double floor, GjShared<Cd>& sh
         const double piv = sh.row[buf][k].re;
         if (!(piv > floor)) return false;  // uniform: every thread reads the same pivot
         const double p = 1.0 / piv;
-        Cd rj[BS];
+        Cd rj[BS], cp[BS];
 #pragma unroll
         for (int v = 0; v < BS; ++v) rj[v] = sh.row[buf][j0 + v];
 #pragma unroll
         for (int u = 0; u < BS; ++u) {
-            const bool row_k = i0 + u == k;
             const Cd ci = sh.col[buf][i0 + u];
-            const Cd cp{ci.re * p, ci.im * p};
+            cp[u] = Cd{ci.re * p, ci.im * p};
+        }
+        // every entry: a_ij - a_ik a_kj / a_kk
+#pragma unroll
+        for (int u = 0; u < BS; ++u)
 #pragma unroll
             for (int v = 0; v < BS; ++v) {
-                const bool col_k = j0 + v == k;
-                // a_ij - a_ik a_kj / a_kk, or the pivot row, column or pivot
-                const double gre = fma(-cp.re, rj[v].re, fma(cp.im, rj[v].im, a[u][v].re));
-                const double gim = fma(-cp.re, rj[v].im, fma(-cp.im, rj[v].re, a[u][v].im));
-                const double rre = rj[v].re * p, rim = rj[v].im * p;
-                const double ore = row_k ? (col_k ? p : rre) : (col_k ? -cp.re : gre);
-                const double oim = row_k ? (col_k ? 0.0 : rim) : (col_k ? -cp.im : gim);
-                a[u][v] = Cd{ore, oim};
+                const double gre =
+                    fma(-cp[u].re, rj[v].re, fma(cp[u].im, rj[v].im, a[u][v].re));
+                const double gim =
+                    fma(-cp[u].re, rj[v].im, fma(-cp[u].im, rj[v].re, a[u][v].im));
+                a[u][v] = Cd{gre, gim};
             }
-        }
+        // then the pivot row (a_kj / a_kk; the pivot 1 / a_kk) and the pivot
+        // column (-a_ik / a_kk) over what the line above wrote there: only
+        // the threads holding row or column k take these branches
+#pragma unroll
+        for (int u = 0; u < BS; ++u)
+            if (i0 + u == k) {
+#pragma unroll
+                for (int v = 0; v < BS; ++v)
+                    a[u][v] = j0 + v == k ? Cd{p, 0.0} : Cd{rj[v].re * p, rj[v].im * p};
+            }
+#pragma unroll
+        for (int v = 0; v < BS; ++v)
+            if (j0 + v == k) {
+#pragma unroll
+                for (int u = 0; u < BS; ++u)
+                    if (i0 + u != k) a[u][v] = Cd{-cp[u].re, -cp[u].im};
+            }
     }
     return true;
 }
