@@ -267,6 +267,25 @@ __device__ __forceinline__ cd cmul(cd a, cd b) {
     return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
+// out[d] = sum_c X[c+d][c] for d < M <= 64 (row stride MP): four lanes per
+// diagonal (4 M <= 256 threads), each summing every fourth entry, combined
+// in a fixed order.
+__device__ __forceinline__ void diagonal_sums(const cd* X, int M, int MP, cd* out, int tid) {
+    if (tid >= 4 * M) return;  // whole groups of four lanes
+    const int d = tid >> 2, part = tid & 3;
+    cd s{0.0, 0.0};
+    for (int c = part; c + d < M; c += 4) {
+        const cd v = X[(c + d) * MP + c];
+        s.re += v.re;
+        s.im += v.im;
+    }
+    s.re += __shfl_xor(s.re, 1);
+    s.im += __shfl_xor(s.im, 1);
+    s.re += __shfl_xor(s.re, 2);
+    s.im += __shfl_xor(s.im, 2);
+    if (part == 0) out[d] = s;
+}
+
 // One workgroup of 256 threads (16 x 16 Gauss-Jordan blocks) per channel;
 // 2 x 2 blocks on 1024 threads measured slower (16-wave barriers, twice
 // the LDS reads).
@@ -300,11 +319,13 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
         buf0[(e / M) * MP + e % M] = cd{(double)p[0] * scale, (double)p[1] * scale};
     }
     __syncthreads();
-    if (tid == 0) {
-        double tr = 0.0;
-        for (int i = 0; i < M; ++i) tr += buf0[i * MP + i].re;
-        s_eps = load > 0.0 ? load * tr / (double)M : 0.0;
-        s_ok = 1;
+    if (tid < 64) {  // the trace, reduced across wave 0 (M <= 64)
+        double tr = tid < M ? buf0[tid * MP + tid].re : 0.0;
+        for (int o = 32; o > 0; o >>= 1) tr += __shfl_xor(tr, o);
+        if (tid == 0) {
+            s_eps = load > 0.0 ? load * tr / (double)M : 0.0;
+            s_ok = 1;
+        }
     }
     __syncthreads();
     // diagonal load, then fold into the carried state
@@ -323,15 +344,7 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
     }
     __syncthreads();
     // Bartlett diagonal sums of R: qb[d] = sum_c R[c+d][c]
-    for (int d = tid; d < M; d += kFoldThreads) {
-        cd s{0.0, 0.0};
-        for (int c = 0; c + d < M; ++c) {
-            const cd v = buf0[(c + d) * MP + c];
-            s.re += v.re;
-            s.im += v.im;
-        }
-        qb[d] = s;
-    }
+    diagonal_sums(buf0, M, MP, qb, tid);
     __syncthreads();
     // Q = R^-1 in place by register-blocked Gauss-Jordan; its pivots are
     // the Cholesky pivots, so R fails the factorisation exactly when a
@@ -354,15 +367,7 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
     }
     const int ok = s_ok;
     if (tid == 0) chol_ok[f] = ok;
-    for (int d = tid; d < M; d += kFoldThreads) {
-        cd s{0.0, 0.0};
-        for (int c = 0; c + d < M; ++c) {
-            const cd v = buf0[(c + d) * MP + c];
-            s.re += v.re;
-            s.im += v.im;
-        }
-        qc[d] = s;
-    }
+    diagonal_sums(buf0, M, MP, qc, tid);
     __syncthreads();
     // a^H X a = q_0 + 2 Re sum_{d>0} q_d e^{j d phi}, phi = k dx sin(theta).
     const double kw = kTwoPi * (double)freqs[f] / kSpeedOfLight;
