@@ -455,8 +455,8 @@ __global__ __launch_bounds__(kThreads) void peaks_kernel(
     __shared__ int wave_count[kThreads / 64];
     __shared__ int total;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    // the spectrum, staged in LDS when it fits (the serial refinement below
-    // reads it three times per candidate)
+    // the spectrum, staged in LDS when it fits (detection and refinement
+    // read it three times per angle)
     const double* Pg = broadband + (size_t)blockIdx.x * n_theta;
     const double* P = Pg;
     if (n_theta <= kPeaksLds) {
