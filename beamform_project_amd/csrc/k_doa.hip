@@ -370,21 +370,41 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
     diagonal_sums(buf0, M, MP, qc, tid);
     __syncthreads();
     // a^H X a = q_0 + 2 Re sum_{d>0} q_d e^{j d phi}, phi = k dx sin(theta).
+    // kAngles angles per thread side by side, so that their recursions
+    // e <- e * step (a dependent chain per angle) overlap.
+    constexpr int kAngles = 3;
     const double kw = kTwoPi * (double)freqs[f] / kSpeedOfLight;
-    for (int q = tid; q < n_theta; q += kFoldThreads) {
-        const double phi = kw * x_step * sin_theta[q];
-        double sn, cs;
-        sincos(phi, &sn, &cs);
-        const cd step{cs, sn};
-        cd e{1.0, 0.0};
-        double pb = qb[0].re, pc = qc[0].re;
-        for (int d = 1; d < M; ++d) {
-            e = cmul(e, step);
-            pb += 2.0 * (qb[d].re * e.re - qb[d].im * e.im);
-            pc += 2.0 * (qc[d].re * e.re - qc[d].im * e.im);
+    for (int q0 = tid; q0 < n_theta; q0 += kAngles * kFoldThreads) {
+        cd step[kAngles], e[kAngles];
+        double pb[kAngles], pc[kAngles];
+#pragma unroll
+        for (int i = 0; i < kAngles; ++i) {
+            const int q = min(q0 + i * kFoldThreads, n_theta - 1);
+            const double phi = kw * x_step * sin_theta[q];
+            double sn, cs;
+            sincos(phi, &sn, &cs);
+            step[i] = cd{cs, sn};
+            e[i] = cd{1.0, 0.0};
+            pb[i] = qb[0].re;
+            pc[i] = qc[0].re;
         }
-        bartlett[(size_t)f * n_theta + q] = pb;
-        capon[(size_t)f * n_theta + q] = (ok && pc > 0.0) ? 1.0 / pc : 0.0;
+        for (int d = 1; d < M; ++d) {
+            const cd b = qb[d], c = qc[d];
+#pragma unroll
+            for (int i = 0; i < kAngles; ++i) {
+                e[i] = cmul(e[i], step[i]);
+                pb[i] += 2.0 * (b.re * e[i].re - b.im * e[i].im);
+                pc[i] += 2.0 * (c.re * e[i].re - c.im * e[i].im);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kAngles; ++i) {
+            const int q = q0 + i * kFoldThreads;
+            if (q < n_theta) {
+                bartlett[(size_t)f * n_theta + q] = pb[i];
+                capon[(size_t)f * n_theta + q] = (ok && pc[i] > 0.0) ? 1.0 / pc[i] : 0.0;
+            }
+        }
     }
 }
 
