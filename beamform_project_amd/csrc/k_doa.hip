@@ -410,7 +410,9 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
 
 // Broadband spectra: sum over channels.  A workgroup covers 64 angles; its
 // kBbParts waves each sum every kBbParts-th channel (f = part, part +
-// kBbParts, ...) and the partial sums are added in part order.
+// kBbParts, ...) into four running sums (channels taken four at a time, so
+// that eight loads are in flight), combined in a fixed order, and the
+// partial sums are added in part order.
 constexpr int kBbParts = 16;
 
 __global__ __launch_bounds__(64 * kBbParts) void broadband_kernel(
@@ -423,10 +425,21 @@ __global__ __launch_bounds__(64 * kBbParts) void broadband_kernel(
     const int q = blockIdx.x * 64 + ql;
     double sc = 0.0, sb = 0.0;
     if (q < n_theta) {
-        for (int f = part; f < F; f += kBbParts) {
-            sc += capon[(size_t)f * n_theta + q];
-            sb += bartlett[(size_t)f * n_theta + q];
+        double c4[4] = {0.0, 0.0, 0.0, 0.0}, b4[4] = {0.0, 0.0, 0.0, 0.0};
+        int f = part;
+        for (; f + 3 * kBbParts < F; f += 4 * kBbParts) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                c4[u] += capon[(size_t)(f + u * kBbParts) * n_theta + q];
+                b4[u] += bartlett[(size_t)(f + u * kBbParts) * n_theta + q];
+            }
         }
+        for (; f < F; f += kBbParts) {
+            c4[0] += capon[(size_t)f * n_theta + q];
+            b4[0] += bartlett[(size_t)f * n_theta + q];
+        }
+        sc = (c4[0] + c4[1]) + (c4[2] + c4[3]);
+        sb = (b4[0] + b4[1]) + (b4[2] + b4[3]);
     }
     part_c[part][ql] = sc;
     part_b[part][ql] = sb;
