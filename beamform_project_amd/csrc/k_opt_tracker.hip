@@ -499,17 +499,29 @@ __global__ __launch_bounds__(kDecideThreads) void opt_decide_kernel(
     float* P = reinterpret_cast<float*>(smem);
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    // four cells per wave at a time, so that their loads are in flight
-    // together
+    // four cells per wave at a time and kSteps channel steps of 64 per
+    // round: every load of a round is issued (at a clamped, in-bounds
+    // address) before the sums, which add in the order f = lane, lane + 64,
+    // ... as one loop over f would
     constexpr int kWaves = kDecideThreads / 64;
+    constexpr int kSteps = 6;
     for (int c0 = wave; c0 < n_cells; c0 += 4 * kWaves) {
         double sum[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int f = lane; f < F; f += 64) {
+        for (int f0 = lane; f0 < F; f0 += 64 * kSteps) {
+            float v[kSteps][4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int cell = c0 + j * kWaves;
-                if (cell < n_cells) sum[j] += (double)part[(size_t)cell * F + f];
-            }
+            for (int k = 0; k < kSteps; ++k)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int f = min(f0 + 64 * k, F - 1);
+                    const int cell = min(c0 + j * kWaves, n_cells - 1);
+                    v[k][j] = part[(size_t)cell * F + f];
+                }
+#pragma unroll
+            for (int k = 0; k < kSteps; ++k)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (f0 + 64 * k < F && c0 + j * kWaves < n_cells) sum[j] += (double)v[k][j];
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
