@@ -313,13 +313,25 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
     const int tid = threadIdx.x;
     const int MM = M * M;
     const double scale = 1.0 / ((double)K * (double)L);
-    // R_block = Rs / (K L)
+    // R_block = Rs / (K L), folded into the carried (unloaded) state
+    double* st = state + (size_t)f * MM * 2;
     for (int e = tid; e < MM; e += kFoldThreads) {
         const int* p = rs + ((size_t)f * MM + e) * 2;
-        buf0[(e / M) * MP + e % M] = cd{(double)p[0] * scale, (double)p[1] * scale};
+        cd v{(double)p[0] * scale, (double)p[1] * scale};
+        if (!first) {
+            v.re = alpha * st[2 * e] + (1.0 - alpha) * v.re;
+            v.im = alpha * st[2 * e + 1] + (1.0 - alpha) * v.im;
+        }
+        st[2 * e] = v.re;
+        st[2 * e + 1] = v.im;
+        buf0[(e / M) * MP + e % M] = v;
     }
     __syncthreads();
-    if (tid < 64) {  // the trace, reduced across wave 0 (M <= 64)
+    // The reference loads each R_block by load * tr(R_block) / M before its
+    // fold, for Capon only.  The trace is linear, so that loaded recursion
+    // equals the unloaded one plus load * tr(R) / M on the diagonal: Bartlett
+    // scans R, Capon inverts R + eps I.
+    if (tid < 64) {  // the trace of the folded R, reduced across wave 0 (M <= 64)
         double tr = tid < M ? buf0[tid * MP + tid].re : 0.0;
         for (int o = 32; o > 0; o >>= 1) tr += __shfl_xor(tr, o);
         if (tid == 0) {
@@ -327,24 +339,10 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
             s_ok = 1;
         }
     }
-    __syncthreads();
-    // diagonal load, then fold into the carried state
-    double* st = state + (size_t)f * MM * 2;
-    for (int e = tid; e < MM; e += kFoldThreads) {
-        const int le = (e / M) * MP + e % M;
-        cd v = buf0[le];
-        if (e / M == e % M) v.re += s_eps;
-        if (!first) {
-            v.re = alpha * st[2 * e] + (1.0 - alpha) * v.re;
-            v.im = alpha * st[2 * e + 1] + (1.0 - alpha) * v.im;
-        }
-        st[2 * e] = v.re;
-        st[2 * e + 1] = v.im;
-        buf0[le] = v;
-    }
-    __syncthreads();
     // Bartlett diagonal sums of R: qb[d] = sum_c R[c+d][c]
     diagonal_sums(buf0, M, MP, qb, tid);
+    __syncthreads();
+    if (tid < M) buf0[tid * MP + tid].re += s_eps;
     __syncthreads();
     // Q = R^-1 in place by register-blocked Gauss-Jordan; its pivots are
     // the Cholesky pivots, so R fails the factorisation exactly when a

@@ -597,8 +597,9 @@ def doa_cpu_baseline(seconds, A, K, M, NTH):
         xpos = (0.3 * np.arange(A)).astype(np.float32)
         theta = np.linspace(-np.pi / 2, np.pi / 2, NTH)
         state = np.zeros((Fs, M, M, 2), np.float64)
+        load = np.zeros(Fs, np.float64)
         t0 = time.perf_counter()
-        O.doa_update(x, K, Fs, A, 0, K, M, 0.98, 1e-3, True, fr, xpos, theta, state)
+        O.doa_update(x, K, Fs, A, 0, K, M, 0.98, 1e-3, True, fr, xpos, theta, state, load)
         return time.perf_counter() - t0
 
     dt = run(1)

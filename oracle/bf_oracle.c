@@ -858,10 +858,18 @@ size_t or_doa_peaks(const double* P, const double* theta, size_t n, size_t max_p
     return count;
 }
 
+/* The reference loads R_block before the fold only for Capon
+ * (src/cpu_opt_beam_tracker.cpp:1879-1892), so its Capon recursion is
+ * R_L <- a R_L + (1 - a)(R_block + eps I) and its Bartlett recursion is the
+ * unloaded R <- a R + (1 - a) R_block.  Since the load only touches the
+ * diagonal, R_L = R + e I with e <- a e + (1 - a) eps: the state carries
+ * the unloaded R and the scalar e per channel, Bartlett scans R and Capon
+ * factors R + e I. */
 int or_doa_update(const uint8_t* x, size_t T, size_t F, size_t A, size_t t0, size_t K,
                   size_t M, double alpha, double load, int first, const float* freqs,
                   const float* xpos, size_t n_theta, const double* theta,
-                  double* R_state, double* capon, double* bartlett, int* chol_ok) {
+                  double* R_state, double* load_state, double* capon, double* bartlett,
+                  int* chol_ok) {
     if (M == 0 || M > A || K == 0 || t0 + K > T) return OR_INVALID;
     const size_t L = A - M + 1;
     int rc = OR_OK;
@@ -877,11 +885,15 @@ int or_doa_update(const uint8_t* x, size_t T, size_t F, size_t A, size_t t0, siz
         or_smooth_forward_i64(Rfull, A, M, Rs);
         const double scale = 1.0 / ((double)K * (double)L);
         for (size_t i = 0; i < 2 * M * M; ++i) Rb[i] = (double)Rs[i] * scale;
-        or_diagonal_load_f64(Rb, M, load);
+        double tr = 0.0;
+        for (size_t i = 0; i < M; ++i) tr += Rb[2 * (i * M + i)];
+        const double eps = load > 0.0 ? load * tr / (double)M : 0.0;
         double* R = R_state + 2 * M * M * f;
         for (size_t i = 0; i < 2 * M * M; ++i)
             R[i] = first ? Rb[i] : alpha * R[i] + (1.0 - alpha) * Rb[i];
+        load_state[f] = first ? eps : alpha * load_state[f] + (1.0 - alpha) * eps;
         memcpy(Lf, R, sizeof(double) * 2 * M * M);
+        for (size_t i = 0; i < M; ++i) Lf[2 * (i * M + i)] += load_state[f];
         const int ok = or_cholesky_f64(Lf, M);
         chol_ok[f] = ok;
         const double k = wave_number(freqs[f]);

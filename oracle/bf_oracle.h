@@ -168,17 +168,21 @@ size_t or_doa_peaks(const double* P, const double* theta, size_t n, size_t max_p
                     double* out);
 
 /* Full C5 update over all channels: covariance -> smoothing -> /(K*L) ->
- * diagonal load -> fold with alpha (first update copies) -> Cholesky ->
- * Capon and Bartlett scan over n_theta angles of a ULA with element x
- * positions xpos (metres), steering exp(-j k x sin(theta))
- * (src/cpu_opt_beam_tracker.cpp:96-106).  R_state: [F][M][M][2] double,
- * carried between calls; first != 0 marks the first update.
+ * fold with alpha (first update copies) -> Bartlett scan of the folded R;
+ * Capon from the Cholesky factor of R + e I, where e is the fold of the
+ * per-update diagonal loads eps = load * tr(R_block) / M (the reference
+ * loads R_block before its fold for Capon only,
+ * src/cpu_opt_beam_tracker.cpp:1879-1892).  Scans cover n_theta angles of a
+ * ULA with element x positions xpos (metres), steering exp(-j k x sin(theta))
+ * (src/cpu_opt_beam_tracker.cpp:96-106).  R_state: [F][M][M][2] double
+ * (unloaded) and load_state: [F] double (e), carried between calls;
+ * first != 0 marks the first update.
  * Outputs: capon[F][n_theta], bartlett[F][n_theta] (double). */
 int or_doa_update(const uint8_t* x, size_t T, size_t F, size_t A, size_t t0,
                   size_t K, size_t M, double alpha, double load, int first,
                   const float* freqs, const float* xpos, size_t n_theta,
-                  const double* theta, double* R_state, double* capon,
-                  double* bartlett, int* chol_ok);
+                  const double* theta, double* R_state, double* load_state,
+                  double* capon, double* bartlett, int* chol_ok);
 
 /* ---- accuracy metrics (BASELINE.md measurement definitions) ------------- */
 /* max_i |g_i - r_i| / max(|r_i|, floor) */

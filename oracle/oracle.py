@@ -88,7 +88,7 @@ def _load():
         "or_doa_peaks": (_sz, [_f64p, _f64p, _sz, _sz, _f64p]),
         "or_doa_update": (C.c_int, [_u8p, _sz, _sz, _sz, _sz, _sz, _sz, C.c_double, C.c_double,
                                     C.c_int, _f32p, _f32p, _sz, _f64p, _f64p, _f64p, _f64p,
-                                    _i32p]),
+                                    _f64p, _i32p]),
         "or_max_rel_err_floored": (C.c_double, [_f32p, _f64p, _sz, C.c_double]),
     }
     for name, (res, args) in sig.items():
@@ -300,15 +300,22 @@ def cholesky_f64(R):
     return L, bool(ok)
 
 
-def doa_update(x, T, F, A, t0, K, M, alpha, load, first, freqs, xpos, theta, R_state):
+def doa_update(x, T, F, A, t0, K, M, alpha, load, first, freqs, xpos, theta, R_state,
+               load_state):
+    """One C5 update.  R_state [F][M][M][2] (the unloaded recursion) and
+    load_state [F] (the folded diagonal load) are float64 arrays updated in
+    place; Capon uses R + load_state * I."""
     nth = len(theta)
     capon = np.zeros((F, nth), np.float64)
     bart = np.zeros((F, nth), np.float64)
     ok = np.zeros(F, np.int32)
+    assert R_state.dtype == np.float64 and R_state.flags.c_contiguous and R_state.size == F * M * M * 2
+    assert load_state.dtype == np.float64 and load_state.flags.c_contiguous and load_state.size == F
     _check(LIB.or_doa_update(np.ascontiguousarray(x, np.uint8).reshape(-1), T, F, A, t0, K, M,
                              alpha, load, int(first), np.ascontiguousarray(freqs, np.float32),
                              np.ascontiguousarray(xpos, np.float32), nth,
-                             np.ascontiguousarray(theta, np.float64), R_state, capon, bart, ok),
+                             np.ascontiguousarray(theta, np.float64), R_state, load_state,
+                             capon, bart, ok),
            "doa_update")
     return capon, bart, ok
 

@@ -64,11 +64,12 @@ def test_spectra_and_recursion_match_fp64_oracle():
     est = DoaEstimator(DoaConfig(FR, n_ant=A, snapshots=K, subarray=M, element_spacing_m=0.25))
     dx = api.DeviceArray.from_numpy(x)
     R_state = np.zeros((F, M, M, 2), np.float64)
+    load_state = np.zeros(F, np.float64)
     for u in range(n_updates):
         est.update(dx, T, u * K)
         cp, bt, ok = est.channel_spectra()
         rc, rb, rok = O.doa_update(x, T, F, A, u * K, K, M, 0.98, 1e-3, u == 0, FR, xpos,
-                                   theta, R_state)
+                                   theta, R_state, load_state)
         assert np.array_equal(ok, rok)
         assert np.max(np.abs(bt - rb) / np.abs(rb).max()) < 1e-9
         assert np.max(np.abs(cp - rc) / np.abs(rc).max()) < 1e-9

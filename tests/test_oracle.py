@@ -365,13 +365,110 @@ def test_doa_update_recovers_single_source():
     theta = np.deg2rad(np.linspace(-90, 90, 721))
     R_state = np.zeros((F, M, M, 2), np.float64)
     capon, bart, ok = O.doa_update(x, K, F, A, 0, K, M, 0.98, 1e-3, True, fr, xpos, theta,
-                                   R_state)
+                                   R_state, np.zeros(F, np.float64))
     assert ok.all()
     for P in (capon.sum(0), bart.sum(0)):
         i = int(np.argmax(P))
         d = O.LIB.or_quadratic_peak_offset(P[i - 1], P[i], P[i + 1])
         est = np.rad2deg(theta[i]) + d * 0.25
         assert abs(est - 17.3) < 0.1
+
+
+def _doa_recursions(x, Fs, A, M, K, n_up, alpha, load, variant="reference"):
+    """Capon / Bartlett operands of the reference's per-window recursion,
+    built here from decoded voltages with numpy (src/cpu_opt_beam_tracker.cpp
+    :1864-1892): R_block = (1/(K L)) sum_l sum_k x_l x_l^H over the forward
+    sub-arrays; Capon folds R_block + load tr(R_block)/M I, Bartlett folds
+    R_block.  `variant` plants a plausible mistake instead."""
+    L = A - M + 1
+    re, im = decode(x)
+    xc = (re + 1j * im).astype(np.complex128)
+    RL = [None] * Fs
+    RU = [None] * Fs
+    out = []
+    for u in range(n_up):
+        blk = xc[u * K:(u + 1) * K]
+        for f in range(Fs):
+            X = blk[:, f, :]
+            Rb = sum(X[:, l:l + M].T @ np.conj(X[:, l:l + M]) for l in range(L))
+            Rb = Rb / (K * (L if variant != "scale_k_only" else 1))
+            eps = load * np.trace(Rb).real / M
+            a_old, a_new = (1 - alpha, alpha) if variant == "alpha_swap" else (alpha, 1 - alpha)
+            if variant == "load_compounds":
+                # the loaded state folded, then loaded again by its own trace
+                RL[f] = Rb.copy() if u == 0 else a_old * RL[f] + a_new * Rb
+                RL[f] = RL[f] + load * np.trace(RL[f]).real / M * np.eye(M)
+            else:
+                Rl = Rb + eps * np.eye(M)
+                RL[f] = Rl if u == 0 else a_old * RL[f] + a_new * Rl
+            RU[f] = Rb if u == 0 else a_old * RU[f] + a_new * Rb
+        out.append(([r.copy() for r in RL],
+                    [(RL[f] if variant == "bartlett_loaded" else RU[f]).copy()
+                     for f in range(Fs)]))
+    return out
+
+
+def test_doa_update_recursion_matches_reference_definition():
+    """or_doa_update over four updates on distinct blocks against the
+    reference's recursion: each block is loaded (Capon only) and then folded
+    with alpha, Bartlett scans the unloaded fold.  The blocks' powers differ
+    by up to 16x, so an alpha <-> 1-alpha swap, a load compounded into the
+    state, a Bartlett taken from the loaded R and a 1/K scale without the
+    sub-array count each move some output far past the tolerance (asserted
+    below on the same data)."""
+    A, M, K, Fs, n_up = 16, 12, 48, 3, 4
+    alpha, load = 0.98, 0.05
+    rng = np.random.default_rng(21)
+    amps = [3.0, 0.75, 2.0, 1.2]
+    v = np.concatenate([a * (rng.standard_normal((K, Fs, A)) + 1j * rng.standard_normal((K, Fs, A)))
+                        for a in amps])
+    re = np.clip(np.rint(v.real), -8, 7).astype(np.int16)
+    im = np.clip(np.rint(v.imag), -8, 7).astype(np.int16)
+    x = ((re & 0xF) | ((im & 0xF) << 4)).astype(np.uint8)
+    T = n_up * K
+    fr = np.array([400e6, 555.5e6, 800e6], np.float32)
+    xpos = (0.25 * np.arange(A)).astype(np.float32)
+    theta = np.linspace(-np.pi / 2, np.pi / 2, 37)
+    k = 2 * np.pi * fr.astype(np.float64) / 299792458.0
+
+    def spectra(RL, RU):
+        cp = np.zeros((Fs, len(theta)))
+        bt = np.zeros((Fs, len(theta)))
+        for f in range(Fs):
+            for q, th in enumerate(theta):
+                a = np.exp(-1j * k[f] * xpos[:M].astype(np.float64) * np.sin(th))
+                cp[f, q] = 1.0 / np.real(a.conj() @ np.linalg.solve(RL[f], a))
+                bt[f, q] = np.real(a.conj() @ RU[f] @ a)
+        return cp, bt
+
+    ref = _doa_recursions(x, Fs, A, M, K, n_up, alpha, load)
+    R_state = np.zeros((Fs, M, M, 2), np.float64)
+    load_state = np.zeros(Fs, np.float64)
+    got = []
+    for u in range(n_up):
+        cp, bt, ok = O.doa_update(x, T, Fs, A, u * K, K, M, alpha, load, u == 0, fr, xpos,
+                                  theta, R_state, load_state)
+        assert ok.all()
+        RL, RU = ref[u]
+        for f in range(Fs):
+            Rc = R_state[f, ..., 0] + 1j * R_state[f, ..., 1]
+            assert np.abs(Rc - RU[f]).max() <= 1e-13 * np.abs(RU[f]).max(), (u, f)
+            assert np.abs(Rc + load_state[f] * np.eye(M) - RL[f]).max() <= \
+                1e-13 * np.abs(RL[f]).max(), (u, f)
+        rc, rb = spectra(RL, RU)
+        assert np.max(np.abs(cp - rc) / np.abs(rc)) < 1e-10, u
+        assert np.max(np.abs(bt - rb) / np.abs(rb)) < 1e-10, u
+        got.append((cp, bt))
+    # each planted mistake is visible on this data at 1e4 x the tolerance
+    for variant in ("alpha_swap", "load_compounds", "bartlett_loaded", "scale_k_only"):
+        bad = _doa_recursions(x, Fs, A, M, K, n_up, alpha, load, variant)
+        worst = 0.0
+        for u in range(n_up):
+            bc, bb = spectra(*bad[u])
+            cp, bt = got[u]
+            worst = max(worst, np.max(np.abs(cp - bc) / np.abs(bc)),
+                        np.max(np.abs(bt - bb) / np.abs(bb)))
+        assert worst > 1e-6, variant
 
 
 def test_opt_tracker_restatement_matches_reference_fixtures():
