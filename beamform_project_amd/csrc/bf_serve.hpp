@@ -21,7 +21,10 @@ inline constexpr int kServeMaxBeams = 64;
 // [t0 + w*S, min(t0 + (w+1)*S, t_end)); window w's integrated [F][B] output
 // goes to out + (w mod out_windows) * F * B.  Beam b points along the
 // trajectory traj[b] = (l0, m0, rate_l, rate_m) evaluated at the window's
-// first spectrum, counted from t0.
+// first spectrum, counted from t0.  w_next is the first window no step has
+// issued yet (0 at admission, advanced by the plan kernel): a request whose
+// start lies before the previous step's head has its early windows issued by
+// the first step after its admission.
 struct ServeSlot {
     int active;
     int B;
@@ -31,7 +34,8 @@ struct ServeSlot {
     long long t_end;
     long long out_windows;
     float* out;
-    long long reserved[2];
+    long long w_next;
+    long long reserved;
     float traj[kServeMaxBeams][4];
 };
 static_assert(sizeof(ServeSlot) == 64 + 16 * kServeMaxBeams, "ServeSlot layout");
@@ -70,7 +74,7 @@ struct ServeLaunch {
     long long ring_mask = 0;
     size_t F = 0, A = 0;
     Geom geo{};
-    const ServeSlot* slots = nullptr;
+    ServeSlot* slots = nullptr;  // the plan kernel advances each slot's window cursor
     ServeCtrl* ctrl = nullptr;
     ServeWork* work = nullptr;
     const long long* heads = nullptr;  // device view of the pinned head ring

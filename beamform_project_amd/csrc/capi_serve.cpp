@@ -18,6 +18,8 @@
 // the ring copy runs on an ingest stream and the steps on a step stream.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -123,6 +125,7 @@ struct Step {
     long long P, H;
     hipEvent_t ev_start, ev_done;
     std::vector<StepReq> reqs;
+    long long min_t;  // first spectrum any of its windows reads (LLONG_MAX: none)
 };
 
 // Nearest-rank percentile of (value, count) samples.
@@ -246,13 +249,13 @@ struct bf_server {
         return (size_t)(e - a);
     }
 
-    // Window range whose last spectrum lies in (P, H]; same rule as the
-    // plan kernel.
-    static void window_range(const Request& r, long long P, long long H, size_t* lo,
-                             size_t* hi) {
+    // Windows a step with head H issues: from the request's cursor (the
+    // first window not yet issued) to the last window ending at or before H;
+    // same rule as the plan kernel.
+    static void window_range(const Request& r, long long H, size_t* lo, size_t* hi) {
         const long long S = (long long)r.S;
         const long long W = (long long)r.windows_total;
-        const long long l = P >= r.t_end ? W : (P <= r.t0 ? 0 : (P - r.t0) / S);
+        const long long l = (long long)r.windows_issued;
         const long long h = H >= r.t_end ? W : (H < r.t0 ? 0 : (H - r.t0) / S);
         *lo = (size_t)l;
         *hi = (size_t)std::max(h, l);
@@ -387,21 +390,28 @@ struct bf_server {
         for (int s : dirty_slots) upload_slot(s);
         dirty_slots.clear();
         admit();
-        const long long H = std::min(head, processed + (long long)cfg.max_step_spectra);
-        if (H <= processed) return 0;
-        Step st{processed, H, get_event(), get_event(), {}};
+        const long long H =
+            std::max(processed, std::min(head, processed + (long long)cfg.max_step_spectra));
+        // A step runs when it releases new spectra or when a request
+        // admitted with a start behind the processed head has windows to
+        // back-fill.
+        Step st{processed, H, nullptr, nullptr, {}, LLONG_MAX};
         size_t n_windows = 0;
         for (size_t i = 0; i < cfg.max_requests; ++i) {
             if (slot_owner[i] == 0) continue;
             Request& r = req(slot_owner[i]);
             size_t lo, hi;
-            window_range(r, processed, H, &lo, &hi);
+            window_range(r, H, &lo, &hi);
             if (hi > lo) {
                 st.reqs.push_back(StepReq{r.id, lo, hi});
-                r.windows_issued = std::max(r.windows_issued, hi);
+                r.windows_issued = hi;
                 n_windows += hi - lo;
+                st.min_t = std::min(st.min_t, r.t0 + (long long)(lo * r.S));
             }
         }
+        if (H <= processed && n_windows == 0) return 0;
+        st.ev_start = get_event();
+        st.ev_done = get_event();
         // Every commit landing spectra in [processed, H): ingest copies are
         // ordered on one stream, but producer commits come from streams of
         // the caller's choosing.  Earlier spectra were covered by earlier
@@ -442,8 +452,11 @@ struct bf_server {
 
     // The step that must finish before spectra [head, head + n) may be
     // written: spectra below X = head + n - R are overwritten, so every
-    // window holding one must have been processed, i.e. a step with
-    // H >= X + max_window.  Null when that has already been harvested.
+    // window holding one must have been issued, i.e. by a step with
+    // H >= X + max_window, and every step reading one must have finished
+    // (a step that back-fills a request admitted with an early start reads
+    // spectra far behind its head).  Steps finish in order, so the later of
+    // the two is the one to wait for.  Null when both have been harvested.
     const Step* overrun_guard(size_t n) const {
         if (n + cfg.max_window > R)
             invalid("writing " + std::to_string(n) +
@@ -451,10 +464,19 @@ struct bf_server {
         const long long X = head + (long long)n - (long long)R;
         if (X <= 0) return nullptr;
         const long long need = X + (long long)cfg.max_window;
-        if (need <= done_head) return nullptr;
-        for (const Step& s : steps)
-            if (s.H >= need) return &s;
-        fail(BF_ERR_LOGIC, "ring overrun: step the server before ingesting more spectra");
+        long long wait_at = -1;
+        if (need > done_head) {
+            for (size_t i = 0; i < steps.size() && wait_at < 0; ++i)
+                if (steps[i].H >= need) wait_at = (long long)i;
+            if (wait_at < 0)
+                fail(BF_ERR_LOGIC, "ring overrun: step the server before ingesting more spectra");
+        }
+        for (size_t i = steps.size(); i-- > 0;)
+            if (steps[i].min_t < X) {
+                wait_at = std::max(wait_at, (long long)i);
+                break;
+            }
+        return wait_at < 0 ? nullptr : &steps[(size_t)wait_at];
     }
 
     // Copies `rows` rows of `row` bytes into ring rows [at, at + rows),

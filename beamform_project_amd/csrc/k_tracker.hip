@@ -709,11 +709,13 @@ __global__ __launch_bounds__(256) void tracker_fp32_kernel(
 
 // ------------------------------------------------------------ serving loop
 // Plan: one thread per request slot reads the step's head, finds the
-// windows whose last spectrum lies in (processed, head] and the work they
-// make; block scans turn the counts into offsets.  The previous head is
-// read before the first barrier and replaced after the last one.
+// windows from the slot's cursor whose last spectrum is at or before the
+// head (a slot admitted with a start behind the previous head gets all its
+// windows held by the ring at once) and the work they make; block scans
+// turn the counts into offsets.  Each thread advances only its own slot's
+// cursor; the step's head is recorded as the processed head.
 __global__ __launch_bounds__(kServeMaxSlots) void serve_plan_kernel(
-    const ServeSlot* __restrict__ slots, ServeCtrl* __restrict__ ctrl,
+    ServeSlot* __restrict__ slots, ServeCtrl* __restrict__ ctrl,
     ServeWork* __restrict__ work, const long long* heads, int F) {
     __shared__ int scan_l[kServeMaxSlots];
     __shared__ int scan_s[kServeMaxSlots];
@@ -722,8 +724,7 @@ __global__ __launch_bounds__(kServeMaxSlots) void serve_plan_kernel(
     // system-scope load: the host wrote this entry just before the launch
     const long long H = __hip_atomic_load(heads + (step & (kServeHeadRing - 1)),
                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    const long long P = ctrl->processed;
-    const ServeSlot& sl = slots[s];
+    ServeSlot& sl = slots[s];
     int n_win = 0;
     int w_lo = 0;
     int n_long = 0;
@@ -733,9 +734,12 @@ __global__ __launch_bounds__(kServeMaxSlots) void serve_plan_kernel(
         const long long t0 = sl.t0;
         const long long te = sl.t_end;
         const long long W = (te - t0 + S - 1) / S;
-        // the last window may be partial and end at te, before t0 + W*S
-        const long long lo = P >= te ? W : (P <= t0 ? 0 : (P - t0) / S);
-        const long long hi = H >= te ? W : (H < t0 ? 0 : (H - t0) / S);
+        // windows from the slot's cursor up to the last one ending at or
+        // before H (the last window may be partial and end at te)
+        const long long lo = sl.w_next;
+        const long long h = H >= te ? W : (H < t0 ? 0 : (H - t0) / S);
+        const long long hi = h > lo ? h : lo;
+        sl.w_next = hi;
         if (hi > lo) {
             n_win = (int)(hi - lo);
             w_lo = (int)lo;

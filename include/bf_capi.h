@@ -385,11 +385,13 @@ bf_status bf_multibeam_tracker_launch(void* stream, const uint8_t* d_packed, flo
  * src/cpu_opt_beam_tracker.cpp, on the GPU).  Per update of `snapshots`
  * spectra and per channel: exact integer covariance R = sum x x^H (int8
  * MFMA, int32 accumulate), forward spatial smoothing over sub-arrays of
- * `subarray` elements, R_block = R_smoothed / (K L) plus diagonal loading
- * diagonal_load * tr(R_block) / M, exponential fold
+ * `subarray` elements, R_block = R_smoothed / (K L), exponential fold
  * R = forgetting * R + (1 - forgetting) * R_block (the first update after
- * create/reset copies), FP64 Cholesky, and Bartlett a^H R a and Capon
- * 1 / (a^H R^-1 a) scans over n_theta angles uniformly spaced in
+ * create/reset copies), Bartlett a^H R a, and Capon 1 / (a^H R_L^-1 a) on
+ * the diagonally loaded R_L = R + diagonal_load * tr(R) / M * I (FP64),
+ * which equals the reference's recursion over R_block loaded by
+ * diagonal_load * tr(R_block) / M before its fold (Capon only,
+ * src/cpu_opt_beam_tracker.cpp:1879-1892); scans over n_theta angles uniformly spaced in
  * [theta_min, theta_max] with a_m = exp(-j 2 pi f / c * m * spacing * sin theta).
  * Broadband spectra sum the channels; peaks are local maxima refined with a
  * 3-point parabola.  n_ant in {32, 64}; 1 <= subarray <= n_ant;
@@ -435,8 +437,8 @@ typedef struct bf_doa_timings {
     double estimate_ms;
 } bf_doa_timings;
 bf_status bf_doa_last_timings(bf_doa* doa, bf_doa_timings* timings);
-/* Checkpoint / resume of the recursion: R [F][M][M][2] FP64 and whether
- * an update has been folded. */
+/* Checkpoint / resume of the recursion: the unloaded R [F][M][M][2] FP64
+ * and whether an update has been folded. */
 bf_status bf_doa_get_state(bf_doa* doa, double* state, int* initialised);
 bf_status bf_doa_set_state(bf_doa* doa, const double* state, int initialised);
 
@@ -703,7 +705,9 @@ bf_status bf_server_acquire(bf_server* server, size_t n_spectra, bf_ring_span* s
 bf_status bf_server_commit(bf_server* server, size_t n_spectra, void* producer_stream);
 /* Admits what fits, then enqueues one step over the spectra ingested since
  * the previous step (at most max_step_spectra); returns the windows it
- * scheduled.  Asynchronous. */
+ * scheduled.  A request admitted with an explicit start behind the
+ * processed head has all its windows that the ring already holds scheduled
+ * by this step, even when no new spectra were ingested.  Asynchronous. */
 bf_status bf_server_step(bf_server* server, size_t* windows_scheduled);
 bf_status bf_server_request_status(bf_server* server, uint64_t id,
                                    bf_request_status* status);

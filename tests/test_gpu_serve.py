@@ -254,6 +254,43 @@ def test_ring_overrun_and_request_validation(stream):
     srv.close()
 
 
+def test_past_start_back_fills_windows(stream):
+    """A request admitted with an explicit start behind the processed head
+    gets every window from its start (the first step after admission issues
+    the ones already in the ring), equal to the multi-beam tracker on the
+    same span.  Spectra 0..799 are ingested and stepped on a 1024-spectrum
+    ring before the requests arrive; the stream then wraps the ring."""
+    reqs = [(7, 10, 600, 0, 22), (24, 320, 900, 50, 23), (3, 16, 400, 650, 24)]
+    srv = SV.TrackerServer(config(ring_spectra=1024))
+    for h in range(0, 800, 100):
+        srv.ingest(stream[h:h + 100])
+        srv.step()
+    ids = [srv.submit(SV.Request(trajectories(B, seed), S, n, start))
+           for B, S, n, start, seed in reqs]
+    srv.step()  # admits; nothing new ingested, the back-filled windows run now
+    assert [srv.status(r)["windows_issued"] for r in ids] == [60, 2, 9]
+    for h in range(800, 1600, 100):
+        srv.ingest(stream[h:h + 100])
+        srv.step()
+    srv.synchronize()
+    for (B, S, n, start, seed), rid in zip(reqs, ids):
+        st = srv.status(rid)
+        assert st["state"] == SV.COMPLETE and st["start"] == start, st
+        got = srv.read(rid, 0, st["windows_total"], B)
+        x = np.ascontiguousarray(stream[start:start + n])
+        W = (n + S - 1) // S
+        dx = api.DeviceArray.from_numpy(x)
+        dpos, dfr, dtraj = (api.DeviceArray.from_numpy(a)
+                            for a in (POS, FR, trajectories(B, seed)))
+        dout = api.DeviceArray.empty((W, F, B), np.float32)
+        TR.multibeam_tracker_launch(None, dx, dout, n, F, A, B, S, dpos, dfr,
+                                    trajectories=dtraj, kernel=K.SPLIT_F16)
+        api.device_synchronize()
+        assert np.array_equal(dout.to_numpy(), got), (B, S, start)
+        srv.release(rid)
+    srv.close()
+
+
 def test_native_load_generator_reports_latency():
     cfg = SV.SimConfig(server=config(ring_spectra=1 << 16, max_requests=64,
                                      hbm_budget_bytes=1 << 30),
