@@ -1,258 +1,225 @@
-// beamformer_amd -- the reference's fixed-grid GPU CLI
-// (tools/beamformer_cuda.cpp) on the MI355X build: packed voltage and
-// weight files in, f32 intensities or int8 codes + parameters + metadata
-// out, with the same flags, output files and metrics CSV columns (backend
-// "hip").  Kernels: direct and tiled (FP32, canonical or tiled weights, as
-// in the reference), split-f16 and f16 (MFMA, canonical weights), auto.
-#include <algorithm>
-#include <chrono>
+// beamformer_amd -- fixed-grid beamforming of a packed voltage file on the
+// GPU.  It answers to the flags of the reference's GPU tool
+// (tools/beamformer_cuda.cpp) and writes the same files: f32 intensities,
+// or int8 codes with their {offset, scale} parameters and the key=value
+// metadata file; an optional metrics CSV gains one row per run with the
+// reference's columns (backend "hip").  Kernels: direct and tiled (FP32 on
+// canonical or tiled weights, as in the reference), split-f16 and f16
+// (MFMA on canonical weights), auto.
+//
+// A run is one Job: the option table fills it, validate() checks it before
+// any file is read, and execute() produces an Outcome that the report and
+// the CSV row are printed from.
 #include <cstddef>
-#include <cstdlib>
 #include <exception>
 #include <filesystem>
-#include <fstream>
 #include <iomanip>
 #include <iostream>
 #include <optional>
+#include <sstream>
 #include <stdexcept>
 #include <string>
-#include <system_error>
 #include <vector>
 
 #include "beamformer/amd.hpp"
 #include "beamformer/amd_io.hpp"
+#include "cli_common.hpp"
 
 namespace {
 
 namespace bfa = beamformer::amd;
-using Clock = std::chrono::steady_clock;
+namespace fs = std::filesystem;
 
-struct Options {
-    std::filesystem::path input, weights, output;
-    std::optional<std::filesystem::path> metrics;
-    std::size_t n_time = 15360, n_ant = 64, n_beams = 5;
-    std::optional<std::size_t> integration_spectra;
-    bool output_int8 = false;
-    std::optional<std::filesystem::path> quantization_parameters, quantization_metadata;
-    std::size_t shard_id = 0;
+enum class Format { Float32, Int8 };
+
+struct Job {
+    fs::path input, weights, output;
+    std::optional<fs::path> metrics, params_file, metadata_file;
+    std::size_t n_time = 15360, n_ant = 64, n_beams = 5, shard = 0;
+    std::optional<std::size_t> integrate;
+    Format format = Format::Float32;
     bfa::Kernel kernel = bfa::Kernel::Direct;
+
+    bfa::Dimensions dims() const { return {n_time, 336, n_ant, n_beams}; }
+    bfa::Dimensions output_dims() const {
+        bfa::Dimensions d = dims();
+        if (integrate) d.n_time = bfa::integrated_time_count(d.n_time, *integrate);
+        return d;
+    }
 };
 
-void print_usage(const char* program) {
-    std::cout << "Usage: " << program << " --input FILE --weights FILE --output FILE [options]\n\n"
-              << "  --n-time N              default: 15360\n"
-              << "  --n-ant N               32, 64, 128 or 256; default: 64\n"
-              << "  --n-beams N             1 to 128; default: 5\n"
-              << "  --kernel NAME           direct, tiled, split-f16, f16 or auto; default: direct\n"
-              << "  --integration-spectra N GPU integration\n"
-              << "  --output-format NAME    float32 or int8; default: float32\n"
-              << "  --quantization-params F int8 parameter output; required for int8\n"
-              << "  --quantization-metadata F int8 metadata output; required for int8\n"
-              << "  --shard-id N            0 or 1; default: 0\n"
-              << "  --metrics FILE          append timing row to CSV\n";
+const std::vector<std::pair<std::string, bfa::Kernel>> kKernels = {
+    {"direct", bfa::Kernel::Direct}, {"tiled", bfa::Kernel::Tiled},
+    {"split-f16", bfa::Kernel::SplitF16}, {"f16", bfa::Kernel::F16}, {"auto", bfa::Kernel::Auto}};
+
+std::string kernel_label(bfa::Kernel k) {
+    for (const auto& [name, value] : kKernels)
+        if (value == k) return name;
+    return "fp32";
 }
 
-const char* require_value(int argc, char** argv, int& i) {
-    if (i + 1 >= argc) throw std::invalid_argument(std::string("missing value after ") + argv[i]);
-    return argv[++i];
+bfcli::OptionTable options_for(Job& j) {
+    using namespace bfcli;
+    OptionTable t;
+    t.add("--input", "FILE", "packed int4 voltages [T][336][A]", path_into(j.input), true)
+        .add("--weights", "FILE", "complex f32 weights (tiled layout for --kernel tiled)",
+             path_into(j.weights), true)
+        .add("--output", "FILE", "f32 intensities, or int8 codes", path_into(j.output), true)
+        .add("--n-time", "N", "spectra; default 15360", size_into(j.n_time))
+        .add("--n-ant", "N", "32, 64, 128 or 256; default 64", size_into(j.n_ant))
+        .add("--n-beams", "N", "1 to 128; default 5", size_into(j.n_beams))
+        .add("--kernel", "NAME", "direct, tiled, split-f16, f16 or auto; default direct",
+             choice_into(j.kernel, kKernels))
+        .add("--integration-spectra", "N", "integrate N spectra per output row",
+             size_into(j.integrate))
+        .add("--output-format", "NAME", "float32 or int8; default float32",
+             choice_into(j.format, std::vector<std::pair<std::string, Format>>{
+                                       {"float32", Format::Float32}, {"int8", Format::Int8}}))
+        .add("--quantization-params", "FILE", "int8 {offset, scale} output (int8 only)",
+             path_into(j.params_file))
+        .add("--quantization-metadata", "FILE", "int8 key=value metadata output (int8 only)",
+             path_into(j.metadata_file))
+        .add("--shard-id", "N", "0 or 1, recorded in the int8 metadata; default 0",
+             size_into(j.shard))
+        .add("--metrics", "FILE", "append one timing row to this CSV", path_into(j.metrics))
+        .help_flags({"--help"});
+    return t;
 }
 
-bfa::Kernel parse_kernel(const std::string& name) {
-    if (name == "direct") return bfa::Kernel::Direct;
-    if (name == "tiled") return bfa::Kernel::Tiled;
-    if (name == "split-f16") return bfa::Kernel::SplitF16;
-    if (name == "f16") return bfa::Kernel::F16;
-    if (name == "auto") return bfa::Kernel::Auto;
-    throw std::invalid_argument("kernel must be direct, tiled, split-f16, f16 or auto");
-}
-
-const char* kernel_name(bfa::Kernel k) {
-    switch (k) {
-    case bfa::Kernel::Direct: return "direct";
-    case bfa::Kernel::Tiled: return "tiled";
-    case bfa::Kernel::SplitF16: return "split-f16";
-    case bfa::Kernel::F16: return "f16";
-    case bfa::Kernel::Auto: return "auto";
-    default: return "fp32";
+void validate(const Job& j, const bfcli::OptionTable& t) {
+    const std::string missing = t.missing_required();
+    if (!missing.empty()) throw std::invalid_argument("required: " + missing);
+    if (j.format == Format::Int8) {
+        if (!j.integrate) throw std::invalid_argument("int8 output needs --integration-spectra");
+        if (!j.params_file || !j.metadata_file)
+            throw std::invalid_argument(
+                "int8 output needs --quantization-params and --quantization-metadata");
     }
+    if (j.shard > 1) throw std::invalid_argument("--shard-id must be 0 or 1");
+    bfa::validate_dimensions(j.dims());
 }
 
-std::size_t parse_size(const char* value, const char* option) {
-    const std::string text(value);
-    std::size_t used = 0;
-    const auto parsed = std::stoull(text, &used);
-    if (used != text.size()) throw std::invalid_argument(std::string("invalid integer for ") + option);
-    return static_cast<std::size_t>(parsed);
+struct Outcome {
+    std::size_t elements = 0, bytes = 0;
+    std::optional<std::size_t> strongest_beam;  // f32 output only
+    bf_timings gpu{};
+    double load_ms = 0, compute_ms = 0, write_ms = 0, total_ms = 0;
+};
+
+// Index of the beam whose intensity summed over every (row, channel) is
+// largest; rows of the [..][B] output are B floats each.
+std::size_t strongest_beam(const std::vector<float>& rows, std::size_t B) {
+    std::vector<double> power(B, 0.0);
+    for (std::size_t r = 0; r + B <= rows.size(); r += B)
+        for (std::size_t b = 0; b < B; ++b) power[b] += rows[r + b];
+    std::size_t best = 0;
+    for (std::size_t b = 1; b < B; ++b)
+        if (power[b] > power[best]) best = b;
+    return best;
 }
 
-Options parse_options(int argc, char** argv) {
-    Options o;
-    for (int i = 1; i < argc; ++i) {
-        const std::string a(argv[i]);
-        if (a == "--help") {
-            print_usage(argv[0]);
-            std::exit(0);
-        } else if (a == "--input") {
-            o.input = require_value(argc, argv, i);
-        } else if (a == "--weights") {
-            o.weights = require_value(argc, argv, i);
-        } else if (a == "--output") {
-            o.output = require_value(argc, argv, i);
-        } else if (a == "--metrics") {
-            o.metrics = require_value(argc, argv, i);
-        } else if (a == "--n-time") {
-            o.n_time = parse_size(require_value(argc, argv, i), "--n-time");
-        } else if (a == "--n-ant") {
-            o.n_ant = parse_size(require_value(argc, argv, i), "--n-ant");
-        } else if (a == "--n-beams") {
-            o.n_beams = parse_size(require_value(argc, argv, i), "--n-beams");
-        } else if (a == "--kernel") {
-            o.kernel = parse_kernel(require_value(argc, argv, i));
-        } else if (a == "--integration-spectra") {
-            o.integration_spectra = parse_size(require_value(argc, argv, i), "--integration-spectra");
-        } else if (a == "--output-format") {
-            const std::string f = require_value(argc, argv, i);
-            if (f != "float32" && f != "int8")
-                throw std::invalid_argument("output format must be float32 or int8");
-            o.output_int8 = f == "int8";
-        } else if (a == "--quantization-params") {
-            o.quantization_parameters = require_value(argc, argv, i);
-        } else if (a == "--quantization-metadata") {
-            o.quantization_metadata = require_value(argc, argv, i);
-        } else if (a == "--shard-id") {
-            o.shard_id = parse_size(require_value(argc, argv, i), "--shard-id");
-        } else {
-            throw std::invalid_argument("unknown option: " + a);
-        }
+Outcome execute(const Job& j) {
+    Outcome o;
+    bfcli::Laps laps;
+    const bfa::Dimensions d = j.dims();
+    const bfa::PackedVoltage voltages = bfa::read_packed_voltage(j.input, d);
+    const std::vector<float> w = j.kernel == bfa::Kernel::Tiled ? bfa::read_tiled_weights(j.weights, d)
+                                                                : bfa::read_weights(j.weights, d);
+    o.load_ms = laps.lap();
+    const bfa::Dimensions od = j.output_dims();
+    if (j.format == Format::Int8) {
+        const bfa::TemporalIntegrationConfig integ{*j.integrate};
+        const auto q = bfa::beamform_packed_quantized_integrated_intensity(voltages, w, d, integ,
+                                                                           j.kernel, &o.gpu);
+        o.compute_ms = laps.lap();
+        bfa::write_quantized_intensities(j.output, q.codes, od);
+        bfa::write_quantization_parameters(*j.params_file, q.parameters, od);
+        bfa::QuantizedOutputMetadata meta;
+        meta.output_dims = od;
+        meta.input_n_time = d.n_time;
+        meta.temporal_integration = integ;
+        meta.shard = bfa::default_shard_descriptors()[j.shard];
+        meta.parameters_file = j.params_file->filename().string();
+        bfa::write_quantized_output_metadata(*j.metadata_file, meta);
+        o.elements = q.codes.size();
+        o.bytes = q.codes.size() + q.parameters.size() * sizeof(float);
+    } else {
+        std::vector<float> out;
+        if (j.integrate)
+            out = bfa::beamform_packed_integrated_intensity(
+                voltages, w, d, bfa::TemporalIntegrationConfig{*j.integrate}, j.kernel, &o.gpu);
+        else
+            out = bfa::beamform_packed_intensity(voltages, w, d, j.kernel, &o.gpu);
+        o.compute_ms = laps.lap();
+        bfa::write_intensities(j.output, out, od);
+        o.elements = out.size();
+        o.bytes = out.size() * sizeof(float);
+        o.strongest_beam = strongest_beam(out, od.n_beams);
     }
-    if (o.input.empty() || o.weights.empty() || o.output.empty())
-        throw std::invalid_argument("--input, --weights, and --output are required");
-    if (o.output_int8 &&
-        (!o.integration_spectra || !o.quantization_parameters || !o.quantization_metadata))
-        throw std::invalid_argument(
-            "int8 output requires --integration-spectra, --quantization-params, and "
-            "--quantization-metadata");
-    if (o.shard_id >= 2) throw std::invalid_argument("--shard-id must be 0 or 1");
+    o.write_ms = laps.lap();
+    o.total_ms = laps.total();
     return o;
 }
 
-double elapsed_ms(Clock::time_point a, Clock::time_point b) {
-    return std::chrono::duration<double, std::milli>(b - a).count();
+// Device time of the beamforming stages (kernel, integration, quantiser).
+double device_ms(const Outcome& o) {
+    return o.gpu.kernel_ms + o.gpu.temporal_integration_ms + o.gpu.quantization_ms;
 }
 
-// Beam with the largest summed intensity (tools/beamformer_cuda.cpp:163-173).
-std::size_t peak_beam(const std::vector<float>& intensity, std::size_t n_beams) {
-    std::vector<double> sum(n_beams, 0.0);
-    for (std::size_t i = 0; i < intensity.size(); ++i) sum[i % n_beams] += intensity[i];
-    return (std::size_t)std::distance(sum.begin(), std::max_element(sum.begin(), sum.end()));
-}
-
-struct Timings {
-    double load_ms = 0, unpack_ms = 0, setup_ms = 0, host_to_device_ms = 0, compute_ms = 0,
-           device_to_host_ms = 0, write_ms = 0, total_ms = 0;
-};
-
-void append_metrics(const std::filesystem::path& path, const bfa::Dimensions& d,
-                    const Timings& t, double rate, double gmac) {
-    std::error_code ec;
-    const bool exists = std::filesystem::exists(path, ec);
-    if (ec) throw std::runtime_error("cannot inspect metrics file: " + path.string());
-    const auto size = exists ? std::filesystem::file_size(path, ec) : 0;
-    if (ec) throw std::runtime_error("cannot inspect metrics file: " + path.string());
-    std::ofstream out(path, std::ios::app);
-    if (!out) throw std::runtime_error("cannot open metrics file: " + path.string());
-    if (!exists || size == 0)
-        out << "backend,n_time,n_freq,n_ant,n_beams,load_ms,unpack_ms,setup_ms,"
-               "host_to_device_ms,compute_ms,device_to_host_ms,write_ms,total_ms,"
-               "output_elements_per_second,complex_gmac_per_second\n";
-    out << std::fixed << std::setprecision(6) << "hip," << d.n_time << ',' << d.n_freq << ','
-        << d.n_ant << ',' << d.n_beams << ',' << t.load_ms << ',' << t.unpack_ms << ','
-        << t.setup_ms << ',' << t.host_to_device_ms << ',' << t.compute_ms << ','
-        << t.device_to_host_ms << ',' << t.write_ms << ',' << t.total_ms << ',' << rate << ','
-        << gmac << '\n';
-    if (!out) throw std::runtime_error("failed to write metrics file: " + path.string());
+void report(const Job& j, const Outcome& o, std::ostream& out) {
+    const bfa::Dimensions d = j.dims(), od = j.output_dims();
+    const double s = device_ms(o) / 1e3;
+    const double elems_per_s = s > 0 ? double(o.elements) / s : 0.0;
+    const double gmac_per_s =
+        s > 0 ? double(d.n_time) * double(d.n_freq) * double(d.n_beams) * double(d.n_ant) / s / 1e9
+              : 0.0;
+    if (j.metrics) {
+        bfcli::CsvAppender csv(*j.metrics);
+        csv.column("backend", "hip").column("n_time", d.n_time).column("n_freq", d.n_freq)
+            .column("n_ant", d.n_ant).column("n_beams", d.n_beams).column("load_ms", o.load_ms)
+            .column("unpack_ms", 0.0).column("setup_ms", o.gpu.setup_ms)
+            .column("host_to_device_ms", o.gpu.host_to_device_ms).column("compute_ms", device_ms(o))
+            .column("device_to_host_ms", o.gpu.device_to_host_ms).column("write_ms", o.write_ms)
+            .column("total_ms", o.total_ms).column("output_elements_per_second", elems_per_s)
+            .column("complex_gmac_per_second", gmac_per_s)
+            .append();
+    }
+    auto layout = [](const bfa::Dimensions& x) {
+        std::ostringstream s;
+        s << "[T=" << x.n_time << "][F=" << x.n_freq << "][B=" << x.n_beams << "]";
+        return s.str();
+    };
+    out << std::fixed << std::setprecision(3) << "HIP beamforming complete: kernel="
+        << kernel_label(j.kernel) << " input_layout=" << layout(d)
+        << " output_layout=" << layout(od) << "\n";
+    const std::vector<std::pair<const char*, double>> times = {
+        {"load_ms", o.load_ms},
+        {"setup_ms", o.gpu.setup_ms},
+        {"host_to_device_ms", o.gpu.host_to_device_ms},
+        {"kernel_ms", device_ms(o)},
+        {"beamformer_ms", o.gpu.kernel_ms},
+        {"temporal_integration_ms", o.gpu.temporal_integration_ms},
+        {"quantization_ms", o.gpu.quantization_ms},
+        {"device_to_host_ms", o.gpu.device_to_host_ms},
+        {"write_ms", o.write_ms},
+        {"total_ms", o.total_ms}};
+    for (std::size_t i = 0; i < times.size(); ++i)
+        out << (i ? " " : "") << times[i].first << '=' << times[i].second;
+    out << "\nkernel_output_elements_per_second=" << elems_per_s
+        << " complex_GMAC_per_second=" << gmac_per_s;
+    if (o.strongest_beam) out << " peak_integrated_beam=" << *o.strongest_beam;
+    out << "\nWrote " << o.bytes << " bytes of output data to " << j.output << "\n";
 }
 
 }  // namespace
 
 int main(int argc, char** argv) {
     try {
-        const Options o = parse_options(argc, argv);
-        const bfa::Dimensions dims{o.n_time, 336, o.n_ant, o.n_beams};
-        bfa::validate_dimensions(dims);
-        const auto t0 = Clock::now();
-        const bfa::PackedVoltage packed = bfa::read_packed_voltage(o.input, dims);
-        const std::vector<float> weights = o.kernel == bfa::Kernel::Tiled
-                                               ? bfa::read_tiled_weights(o.weights, dims)
-                                               : bfa::read_weights(o.weights, dims);
-        const auto t_load = Clock::now();
-        bf_timings gt{};
-        bfa::Dimensions out_dims = dims;
-        if (o.integration_spectra)
-            out_dims.n_time = bfa::integrated_time_count(dims.n_time, *o.integration_spectra);
-        std::size_t out_bytes = 0, out_elements = 0;
-        std::optional<std::size_t> peak;
-        Clock::time_point t_compute;
-        if (o.output_int8) {
-            const bfa::TemporalIntegrationConfig cfg{*o.integration_spectra};
-            const auto q = bfa::beamform_packed_quantized_integrated_intensity(
-                packed, weights, dims, cfg, o.kernel, &gt);
-            t_compute = Clock::now();
-            bfa::write_quantized_intensities(o.output, q.codes, out_dims);
-            bfa::write_quantization_parameters(*o.quantization_parameters, q.parameters,
-                                               out_dims);
-            bfa::QuantizedOutputMetadata meta;
-            meta.output_dims = out_dims;
-            meta.input_n_time = dims.n_time;
-            meta.temporal_integration = cfg;
-            meta.shard = bfa::default_shard_descriptors()[o.shard_id];
-            meta.parameters_file = o.quantization_parameters->filename().string();
-            bfa::write_quantized_output_metadata(*o.quantization_metadata, meta);
-            out_elements = q.codes.size();
-            out_bytes = q.codes.size() + q.parameters.size() * sizeof(float);
-        } else {
-            const std::vector<float> intensity =
-                o.integration_spectra
-                    ? bfa::beamform_packed_integrated_intensity(
-                          packed, weights, dims,
-                          bfa::TemporalIntegrationConfig{*o.integration_spectra}, o.kernel, &gt)
-                    : bfa::beamform_packed_intensity(packed, weights, dims, o.kernel, &gt);
-            t_compute = Clock::now();
-            bfa::write_intensities(o.output, intensity, out_dims);
-            out_elements = intensity.size();
-            out_bytes = intensity.size() * sizeof(float);
-            peak = peak_beam(intensity, out_dims.n_beams);
-        }
-        const auto t_write = Clock::now();
-        Timings t;
-        t.load_ms = elapsed_ms(t0, t_load);
-        t.setup_ms = gt.setup_ms;
-        t.host_to_device_ms = gt.host_to_device_ms;
-        t.compute_ms = gt.kernel_ms + gt.temporal_integration_ms + gt.quantization_ms;
-        t.device_to_host_ms = gt.device_to_host_ms;
-        t.write_ms = elapsed_ms(t_compute, t_write);
-        t.total_ms = elapsed_ms(t0, t_write);
-        const double secs = t.compute_ms / 1000.0;
-        const double raw = double(dims.n_time * dims.n_freq * dims.n_beams);
-        const double rate = secs > 0 ? double(out_elements) / secs : 0.0;
-        const double gmac = secs > 0 ? raw * double(dims.n_ant) / secs / 1e9 : 0.0;
-        if (o.metrics) append_metrics(*o.metrics, dims, t, rate, gmac);
-        std::cout << std::fixed << std::setprecision(3)
-                  << "HIP beamforming complete: kernel=" << kernel_name(o.kernel)
-                  << " input_layout=[T=" << dims.n_time << "][F=" << dims.n_freq
-                  << "][B=" << dims.n_beams << "] output_layout=[T=" << out_dims.n_time
-                  << "][F=" << out_dims.n_freq << "][B=" << out_dims.n_beams << "]\n"
-                  << "load_ms=" << t.load_ms << " setup_ms=" << t.setup_ms
-                  << " host_to_device_ms=" << t.host_to_device_ms
-                  << " kernel_ms=" << t.compute_ms << " beamformer_ms=" << gt.kernel_ms
-                  << " temporal_integration_ms=" << gt.temporal_integration_ms
-                  << " quantization_ms=" << gt.quantization_ms
-                  << " device_to_host_ms=" << t.device_to_host_ms
-                  << " write_ms=" << t.write_ms << " total_ms=" << t.total_ms << "\n"
-                  << "kernel_output_elements_per_second=" << rate
-                  << " complex_GMAC_per_second=" << gmac;
-        if (peak) std::cout << " peak_integrated_beam=" << *peak;
-        std::cout << "\nWrote " << out_bytes << " bytes of output data to " << o.output << "\n";
+        Job job;
+        bfcli::OptionTable table = options_for(job);
+        if (!table.parse(argc, argv, bfcli::Unknown::Reject, std::cout)) return 0;
+        validate(job, table);
+        report(job, execute(job), std::cout);
         return 0;
     } catch (const std::exception& e) {
         std::cerr << "beamformer_amd: " << e.what() << "\n";

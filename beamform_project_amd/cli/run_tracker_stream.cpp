@@ -22,43 +22,55 @@
 
 #include "beamformer/amd.hpp"
 #include "beamformer/amd_io.hpp"
+#include "cli_common.hpp"
 
 namespace {
 
 namespace bfa = beamformer::amd;
 
-void print_usage(const char* prog) {
-    std::cout << "Usage: " << prog << " [options]\n"
-              << "  --engine ENGINE       amd_split (default), amd_f16, amd_fp32; the\n"
-              << "                        reference's cuda_* names run amd_split\n"
-              << "  --n-time N            number of time samples\n"
-              << "  --n-freq N            number of frequency channels (default 336)\n"
-              << "  --n-ant N             number of antenna elements (32, 64, 128, 256)\n"
-              << "  --spectra N           integration spectra per window (default 320)\n"
-              << "  --source-l0 L0        steering start l0\n"
-              << "  --source-m0 M0        steering start m0\n"
-              << "  --source-dl DL        steering rate dl/sample\n"
-              << "  --source-dm DM        steering rate dm/sample\n"
-              << "  --input FILE          input packed voltage binary file\n"
-              << "  --output FILE         output float32 intensity binary file\n";
-}
+struct Run {
+    std::string engine = "amd_split";
+    std::size_t n_time = 15360, n_freq = 336, n_ant = 64, spectra = 320;
+    float l0 = 0.0F, m0 = 0.0F, dl = 0.0F, dm = 0.0F;
+    std::string input, output;
+};
 
-bool starts_with(const std::string& s, const char* p) { return s.rfind(p, 0) == 0; }
+bfcli::OptionTable options_for(Run& r) {
+    using namespace bfcli;
+    OptionTable t;
+    // lenient numbers (a leading number is read, as std::stoull / std::stof do)
+    t.add("--engine", "ENGINE", "amd_split (default), amd_f16, amd_fp32; the reference's cuda_* "
+                                "names run amd_split", text_into(r.engine))
+        .add("--n-time", "N", "number of time samples", size_into(r.n_time, false))
+        .add("--n-freq", "N", "number of frequency channels (default 336)", size_into(r.n_freq, false))
+        .add("--n-ant", "N", "number of antenna elements (32, 64, 128, 256)", size_into(r.n_ant, false))
+        .add("--spectra", "N", "integration spectra per window (default 320)",
+             size_into(r.spectra, false))
+        .add("--source-l0", "L0", "steering start l0", float_into(r.l0))
+        .add("--source-m0", "M0", "steering start m0", float_into(r.m0))
+        .add("--source-dl", "DL", "steering rate dl/sample", float_into(r.dl))
+        .add("--source-dm", "DM", "steering rate dm/sample", float_into(r.dm))
+        .add("--input", "FILE", "input packed voltage binary file", text_into(r.input))
+        .add("--output", "FILE", "output float32 intensity binary file", text_into(r.output))
+        .help_flags({"--help", "-h"});
+    return t;
+}
 
 // Engine name -> kernel; throws for names this build does not provide.
 bfa::Kernel engine_kernel(const std::string& engine) {
-    if (engine == "amd_split" || engine == "auto") return bfa::Kernel::SplitF16;
+    auto prefixed = [&](const char* p) { return engine.rfind(p, 0) == 0; };
+    if (engine == "amd_split" || engine == "auto" || prefixed("cuda_")) return bfa::Kernel::SplitF16;
     if (engine == "amd_f16") return bfa::Kernel::F16;
     if (engine == "amd_fp32") return bfa::Kernel::Fp32;
-    if (starts_with(engine, "cuda_")) return bfa::Kernel::SplitF16;
-    if (starts_with(engine, "cpu_"))
+    if (prefixed("cpu_"))
         throw std::invalid_argument("engine " + engine +
                                     " is the reference's CPU tracker, not part of this build");
     throw std::invalid_argument("Unknown engine: " + engine);
 }
 
-// direction_from_lm (src/geometry.cpp:97-106), float arithmetic.
-void direction_from_lm(float l, float m, float out[3]) {
+// Unit vector of direction cosines (l, m), as src/geometry.cpp:97-106
+// defines it, in float arithmetic.
+void unit_direction(float l, float m, float out[3]) {
     if (!std::isfinite(l) || !std::isfinite(m))
         throw std::invalid_argument("direction cosines l and m must be finite");
     const float t2 = l * l + m * m;
@@ -71,51 +83,30 @@ void direction_from_lm(float l, float m, float out[3]) {
 }  // namespace
 
 int main(int argc, char** argv) {
-    std::string engine = "amd_split";
-    std::size_t n_time = 15360, n_freq = 336, n_ant = 64, spectra = 320;
-    float l0 = 0.0F, m0 = 0.0F, dl = 0.0F, dm = 0.0F;
-    std::string input_file, output_file;
+    Run run;
+    bfcli::OptionTable table = options_for(run);
     try {
-        for (int i = 1; i < argc; ++i) {
-            const std::string a = argv[i];
-            const bool more = i + 1 < argc;
-            if (a == "--engine" && more) engine = argv[++i];
-            else if (a == "--n-time" && more) n_time = std::stoull(argv[++i]);
-            else if (a == "--n-freq" && more) n_freq = std::stoull(argv[++i]);
-            else if (a == "--n-ant" && more) n_ant = std::stoull(argv[++i]);
-            else if (a == "--spectra" && more) spectra = std::stoull(argv[++i]);
-            else if (a == "--source-l0" && more) l0 = std::stof(argv[++i]);
-            else if (a == "--source-m0" && more) m0 = std::stof(argv[++i]);
-            else if (a == "--source-dl" && more) dl = std::stof(argv[++i]);
-            else if (a == "--source-dm" && more) dm = std::stof(argv[++i]);
-            else if (a == "--input" && more) input_file = argv[++i];
-            else if (a == "--output" && more) output_file = argv[++i];
-            else if (a == "--help" || a == "-h") {
-                print_usage(argv[0]);
-                return 0;
-            }
-        }
-        if (input_file.empty() || output_file.empty()) {
-            std::cerr << "Error: --input and --output are required.\n";
-            print_usage(argv[0]);
+        if (!table.parse(argc, argv, bfcli::Unknown::Skip, std::cout)) return 0;
+        if (run.input.empty() || run.output.empty()) {
+            std::cerr << "Error: --input and --output are required.\n" << table.usage(argv[0]);
             return 1;
         }
-        const bfa::Kernel kernel = engine_kernel(engine);
-        const bfa::Dimensions dims{n_time, n_freq, n_ant, 1};
+        const bfa::Kernel kernel = engine_kernel(run.engine);
+        const bfa::Dimensions dims{run.n_time, run.n_freq, run.n_ant, 1};
         bfa::validate_dimensions(dims);
         bfa::TrackerConfig cfg;
-        direction_from_lm(l0, m0, cfg.direction_start);
-        cfg.direction_rate_per_sample[0] = dl;
-        cfg.direction_rate_per_sample[1] = dm;
-        cfg.integration_spectra = spectra;
+        unit_direction(run.l0, run.m0, cfg.direction_start);
+        cfg.direction_rate_per_sample[0] = run.dl;
+        cfg.direction_rate_per_sample[1] = run.dm;
+        cfg.integration_spectra = run.spectra;
         // Inputs and arguments are validated before the GPU is touched.
-        const bfa::PackedVoltage packed = bfa::read_packed_voltage(input_file, dims);
+        const bfa::PackedVoltage packed = bfa::read_packed_voltage(run.input, dims);
         bfa::TrackerExecution exec;
         exec.kernel = kernel;
         const std::vector<float> intensity = bfa::beam_tracker_into(packed, dims, cfg, exec);
-        bfa::write_intensities(output_file, intensity, dims);
-        std::cout << "Successfully ran " << engine << " on " << input_file << " -> wrote "
-                  << output_file << " (" << intensity.size() * sizeof(float) << " bytes)\n";
+        bfa::write_intensities(run.output, intensity, dims);
+        std::cout << "Successfully ran " << run.engine << " on " << run.input << " -> wrote "
+                  << run.output << " (" << intensity.size() * sizeof(float) << " bytes)\n";
         return 0;
     } catch (const std::exception& e) {
         std::cerr << "Error: " << e.what() << "\n";
