@@ -344,7 +344,11 @@ struct bf_server {
     void admit() {
         while (!pending.empty()) {
             Request& r = req(pending.front());
-            if (r.start >= 0 && r.start < head - (long long)R + (long long)cfg.max_window) {
+            // Spectra below head - R have been overwritten.  Every window of a
+            // request admitted here that ends at or before the step's head is
+            // issued by this same step, before any later ingest can reach it
+            // (the overrun guard waits for steps that read old spectra).
+            if (r.start >= 0 && r.start < head - (long long)R) {
                 r.state = kRejected;
                 r.reject_reason = "start spectrum is no longer held by the ring";
                 ++n_rejected;

@@ -288,6 +288,11 @@ def test_past_start_back_fills_windows(stream):
         api.device_synchronize()
         assert np.array_equal(dout.to_numpy(), got), (B, S, start)
         srv.release(rid)
+    # head 1600 on a 1024-spectrum ring: spectrum 576 is the oldest held.
+    late = [srv.submit(SV.Request(trajectories(2, 25), 10, 100, s0)) for s0 in (575, 576)]
+    srv.step()
+    assert [srv.status(r)["state"] for r in late] == [SV.REJECTED, SV.ACTIVE]
+    assert srv.status(late[1])["windows_issued"] == 10
     srv.close()
 
 
