@@ -108,6 +108,108 @@ __global__ __launch_bounds__(256) void quantize_kernel(const float* __restrict__
     for (int e = threadIdx.x; e < n; e += 256) codes[base + e] = ctile[e];
 }
 
+// Min, max and a non-finite flag reduced over the 16 lanes that share
+// (lane & 3), i.e. over one chunk's 16 channels.
+__device__ __forceinline__ void chunk_reduce(float& mn, float& mx, int& bad) {
+#pragma unroll
+    for (int off = 4; off <= 32; off <<= 1) {
+        mn = fminf(mn, __shfl_xor(mn, off));
+        mx = fmaxf(mx, __shfl_xor(mx, off));
+        bad |= __shfl_xor(bad, off);
+    }
+}
+
+// Vector form for B a multiple of 16 (16-byte aligned rows): one wave per
+// (window t, 16-channel tile fi, group of four 16-beam chunks).  Lane l
+// holds channel f0 + l/4 of chunk 4g + l%4: its 16 beams are one 64-byte
+// run, read with four 16-byte loads, and its 16 codes leave in one 16-byte
+// store.  The chunk's min/max is the lane's own over its 16 values, then
+// a butterfly over the 16 lanes of the chunk; every value is quantised
+// with the same separately rounded operations as quantize_kernel.
+__global__ __launch_bounds__(256) void quantize_vec16_kernel(const float* __restrict__ in,
+                                                             int8_t* __restrict__ codes,
+                                                             float2* __restrict__ params,
+                                                             int F, int B, int ft, int bt,
+                                                             int groups, long n_waves) {
+#pragma clang fp contract(off)
+    const long wid = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wid >= n_waves) return;  // whole waves only
+    const int lane = threadIdx.x & 63;
+    const int g = (int)(wid % groups);
+    const long rest = wid / groups;
+    const int fi = (int)(rest % ft);
+    const long t = rest / ft;
+    const int f = fi * 16 + (lane >> 2);
+    const int bi = g * 4 + (lane & 3);
+    const bool ok = f < F && bi < bt;
+    const size_t off = ((size_t)t * F + f) * B + (size_t)bi * 16;
+    float v[16];
+    float mn = INFINITY, mx = -INFINITY;
+    int bad = 0;
+    if (ok) {
+        typedef float f32x4 __attribute__((ext_vector_type(4)));
+        const f32x4* p = reinterpret_cast<const f32x4*>(in + off);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const f32x4 u = __builtin_nontemporal_load(p + q);
+            v[4 * q] = u.x;
+            v[4 * q + 1] = u.y;
+            v[4 * q + 2] = u.z;
+            v[4 * q + 3] = u.w;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (!isfinite(v[k])) {
+                bad = 1;
+            } else {
+                mn = fminf(mn, v[k]);
+                mx = fmaxf(mx, v[k]);
+            }
+        }
+    }
+    chunk_reduce(mn, mx, bad);
+    constexpr float out_min = -127.0f - 0.5f;
+    constexpr float out_max = 127.0f + 0.5f;
+    constexpr float out_range = out_max - out_min;
+    float offset = 0.0f, scale = 0.0f;
+    if (!bad && isfinite(mn) && isfinite(mx)) {
+        const float span = mx - mn;
+        scale = span / out_range;
+        const float shift = out_min * scale;
+        offset = mn - shift;
+        if (!isfinite(offset) || !isfinite(scale)) {
+            offset = 0.0f;
+            scale = 0.0f;
+        }
+    }
+    if (!ok) return;
+    if ((lane >> 2) == 0) params[((size_t)t * ft + fi) * bt + bi] = make_float2(offset, scale);
+    // The code is rint of the correctly rounded quotient d / scale.  With
+    // scale >= 2^-100 the quotient is formed as d * rcp(scale) (relative
+    // error below 2^-22.2; |d / scale| <= 128, so within 2.3e-5 of the
+    // exact quotient, and the correctly rounded one is within 1.5e-5 of
+    // that).  Where this q lies at least 2^-12 from every half-integer, the
+    // correctly rounded quotient lies strictly on the same side of each
+    // half-integer and is not a tie, so both round to the same integer;
+    // elsewhere, and for smaller scales, the correctly rounded division is
+    // taken.
+    const bool fast = scale >= 0x1p-100f;
+    const float rcp = __builtin_amdgcn_rcpf(fast ? scale : 1.0f);
+    uint32_t packed[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const float d = v[k] - offset;
+        float q = d * rcp;
+        const float fr = q - floorf(q);
+        if (!fast || fabsf(fr - 0.5f) < 0x1p-12f) q = d / scale;
+        const float r = fminf(fmaxf(__builtin_rintf(q), -127.0f), 127.0f);
+        int code = scale != 0.0f ? (int)r : 0;
+        code = isfinite(v[k]) ? code : -128;
+        packed[k >> 2] |= ((uint32_t)code & 0xFFu) << (8 * (k & 3));
+    }
+    *reinterpret_cast<uint4*>(codes + off) = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+}
+
 __device__ __forceinline__ uint64_t splitmix(uint64_t key, uint64_t seed) {
     uint64_t z = key + (seed + 1u) * 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -168,6 +270,18 @@ void launch_quantize(hipStream_t stream, const float* in, int8_t* codes, float* 
     if (B > kMaximumBeams) invalid("n_beams must be between 1 and 128");
     if (n_blocks > (size_t)UINT32_MAX)
         fail(BF_ERR_OVERFLOW, "quantization grid exceeds the supported size");
+    const bool vec = B % 16 == 0 && reinterpret_cast<uintptr_t>(in) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(codes) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(params) % 8 == 0;
+    if (vec) {
+        const size_t groups = ceil_div(bt, 4);
+        const size_t n_waves = Tw * ft * groups;
+        hipLaunchKernelGGL(quantize_vec16_kernel, dim3((unsigned)ceil_div(n_waves, 4)),
+                           dim3(256), 0, stream, in, codes, reinterpret_cast<float2*>(params),
+                           (int)F, (int)B, (int)ft, (int)bt, (int)groups, (long)n_waves);
+        hip_check(hipGetLastError(), "quantize_vec16_kernel launch");
+        return;
+    }
     hipLaunchKernelGGL(quantize_kernel, dim3((unsigned)n_blocks), dim3(256), 0, stream, in,
                        codes, reinterpret_cast<float2*>(params), (int)F, (int)B, (int)ft,
                        (int)bt);

@@ -228,6 +228,43 @@ def test_quantizer_special_values():
     assert codes.to_numpy()[1, 3, 5] == -128
 
 
+@pytest.mark.parametrize("B", [16, 48, 128])
+def test_quantizer_16_beam_rows(B):
+    """B a multiple of 16 takes the 16-byte-row quantiser, whose quotients
+    are formed by a reciprocal and re-divided near rounding boundaries: the
+    codes equal the CPU quantiser's on random chunks, on chunks whose
+    quotients are exact half-integers (ties to even), on a chunk with a
+    subnormal scale, and on constant, NaN and inf chunks."""
+    Tw = 7
+    rng = np.random.default_rng(B)
+    v = rng.uniform(0, 1000, (Tw, F, B)).astype(np.float32)
+    v[1] = rng.exponential(50.0, (F, B)).astype(np.float32)
+    # min 0, max 255 -> scale 1, offset 127.5: every integer value is a tie
+    v[2, :16, :16] = rng.integers(0, 256, (16, 16)).astype(np.float32)
+    v[2, 0, 0], v[2, 1, 1] = 0.0, 255.0
+    # span 510 -> scale 2: odd integers give ties, even ones integers
+    v[2, 16:32, :16] = rng.integers(0, 511, (16, 16)).astype(np.float32)
+    v[2, 16, 0], v[2, 17, 1] = 0.0, 510.0
+    v[3, :16, :16] = rng.uniform(0, 1e-38, (16, 16)).astype(np.float32)  # subnormal scale
+    v[4, :16, :16] = 42.0          # constant chunk
+    v[5, 3, 5] = np.nan            # NaN chunk
+    v[6, 20, B - 1] = np.inf       # inf chunk
+    v[6, 40, 2] = -np.inf
+    d_in = api.DeviceArray.from_numpy(v)
+    codes = api.DeviceArray.empty((Tw, F, B), np.int8)
+    npar = api.quantization_parameter_count(api.Dimensions(Tw, F, 32, B))
+    params = api.DeviceArray.empty((npar, 2), np.float32)
+    api.launch_quantize_integrated_intensity(None, d_in, codes, params,
+                                             api.Dimensions(Tw, F, 32, B))
+    api.device_synchronize()
+    rc, rp = O.quantize(v, Tw, F, B)
+    assert np.array_equal(params.to_numpy(), rp)
+    assert np.array_equal(codes.to_numpy(), rc)
+    got = codes.to_numpy()
+    assert got[5, 3, 5] == -128 and got[6, 20, B - 1] == -128 and got[6, 40, 2] == -128
+    assert np.all(got[4, :16, :16] == 0)
+
+
 # ------------------------------------------------------------- workspace
 def test_workspace_contract():
     cap = api.Dimensions(128, F, 64, 16)
