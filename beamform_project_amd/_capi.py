@@ -95,6 +95,8 @@ SIGNATURES = {
     "bf_launch_packed_integrated_beamformer_masked": [C.c_int, _vp, _vp, _vp, _vp, _vp, _pd,
                                                       _sz],
     "bf_launch_quantize_integrated_intensity": [_vp, _vp, _vp, _vp, _pd],
+    "bf_launch_packed_quantized_integrated_beamformer_noalloc": [C.c_int, _vp, _vp, _vp, _vp,
+                                                                 _vp, _pd, _sz],
     "bf_workspace_create": [_pd, C.c_int, _sz, C.c_int, C.POINTER(_vp)],
     "bf_workspace_destroy": [_vp],
     "bf_workspace_setup_ms": [_vp, _pdbl],

@@ -251,6 +251,21 @@ def launch_quantize_integrated_intensity(stream, integrated_intensity, quantized
          _ptr(quantized_intensity), _ptr(parameters), C.byref(d))
 
 
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
+
+def launch_packed_quantized_integrated_beamformer_noalloc(
+        kernel: Kernel, stream, packed_voltage, weights, quantized_intensity, parameters,
+        dims: Dimensions, integration_spectra: int) -> None:
+    """Beamform + integrate + int8-quantise with no device allocation: the
+    f32 intermediate lives in the library's per-GPU scratch of
+    NOALLOC_SCRATCH_BYTES (calls overlapping on one GPU race)."""
+    d = dims.c()
+    call("bf_launch_packed_quantized_integrated_beamformer_noalloc", int(kernel), _sh(stream),
+         _ptr(packed_voltage), _ptr(weights), _ptr(quantized_intensity), _ptr(parameters),
+         C.byref(d), integration_spectra)
+
+
 def autotune_packed_beamformer(kernel: Kernel, stream, packed_voltage, weights, output,
                                dims: Dimensions, integration_spectra: int = 0,
                                repeats: int = 3) -> list:

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "bf_common.hpp"
@@ -251,6 +252,36 @@ void validate_frame_shape(const bf_frame_view& v, int type, size_t rank,
     if (!same) invalid(std::string(label) + " shape or dtype does not match");
 }
 
+// ------------------------------------------------ non-allocating launcher
+// The f32 integrated intermediate of bf_launch_packed_quantized_integrated_
+// beamformer_noalloc lives in this file-scope device array: every GPU the
+// library runs on holds its own copy, at its own address, until the process
+// ends.  Host code never passes `scratch` itself to a kernel (in host code
+// the name is a stand-in, not a device address); it uses the address
+// hipGetSymbolAddress returns, once per device, with that device current.
+__device__ __align__(256) float scratch[BF_NOALLOC_SCRATCH_BYTES / sizeof(float)];
+
+constexpr int kMaxScratchDevices = 256;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxScratchDevices];
+
+float* scratch_for_current_device() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        fail(BF_ERR_RUNTIME, "device ordinal " + std::to_string(dev) +
+                                 " is beyond the devices the scratch table covers");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                  "hipGetSymbolAddress(scratch)");
+        if (p == nullptr) fail(BF_ERR_RUNTIME, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace
 }  // namespace bfhip
 
@@ -390,6 +421,32 @@ bf_status bf_launch_quantize_integrated_intensity(void* stream, const float* int
         require_ptr(params, "parameters");
         launch_quantize(static_cast<hipStream_t>(stream), integrated, codes, params, d.n_time,
                         d.n_freq, d.n_beams);
+    });
+}
+
+bf_status bf_launch_packed_quantized_integrated_beamformer_noalloc(
+    int kernel, void* stream, const uint8_t* packed, const float* weights, int8_t* codes,
+    float* params, const bf_dims* dims, size_t S) {
+    return guarded([&] {
+        validate_kernel(kernel);
+        const bf_dims& d = deref_dims(dims);
+        validate_dimensions(d);
+        validate_integration(kernel, S);
+        require_ptr(codes, "quantized_intensity");
+        require_ptr(params, "parameters");
+        const bf_dims id = integrated_dims_of(d, S);
+        const size_t need = id.n_time * id.n_freq * id.n_beams * sizeof(float);
+        if (need > BF_NOALLOC_SCRATCH_BYTES)
+            invalid("the integrated intermediate of " + std::to_string(need) +
+                    " bytes exceeds the " + std::to_string(BF_NOALLOC_SCRATCH_BYTES) +
+                    "-byte scratch of the non-allocating launcher; split the frame or use "
+                    "bf_launch_packed_integrated_beamformer with a caller buffer");
+        float* mid = scratch_for_current_device();
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        // Every window of the intermediate is written by the beamformer
+        // before the quantiser reads it, so nothing needs clearing.
+        run_beamformer(kernel, s, packed, weights, mid, d, S);
+        launch_quantize(s, mid, codes, params, id.n_time, id.n_freq, id.n_beams);
     });
 }
 

@@ -119,6 +119,25 @@ bf_status bf_launch_quantize_integrated_intensity(void* stream,
                                                   float* parameters,
                                                   const bf_dims* integrated_dims);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): beamform + integrate + int8-quantise device-resident
+ * voltages straight into caller-owned codes [W][F][B] and {offset, scale}
+ * parameters, with the f32 integrated intermediate in a file-scope device
+ * array of BF_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends).
+ * Async on `stream`; calls no hipMalloc/hipFree.  Results equal
+ * bf_launch_packed_integrated_beamformer followed by
+ * bf_launch_quantize_integrated_intensity (and the workspace's int8 device
+ * frame) bit for bit.  Rejects, before launch, problems whose intermediate
+ * W*F*B*4 bytes exceeds the scratch (BF_ERR_INVALID_ARGUMENT).  Every call
+ * on one GPU shares that GPU's scratch: calls that overlap on one GPU (on
+ * different streams, or from different threads) race. */
+#define BF_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+bf_status bf_launch_packed_quantized_integrated_beamformer_noalloc(
+    int kernel, void* stream, const uint8_t* packed_voltage, const float* weights,
+    int8_t* quantized_intensity, float* parameters, const bf_dims* dims,
+    size_t integration_spectra);
+
 /* Which kernel AUTO resolves to for these dimensions (a bf_kernel value). */
 /* The stage launchers with a loss mask: one validity byte per [t][f] frame
  * (device memory, 0 = lost; include/beamformer/formats.hpp:22-32).  A lost
