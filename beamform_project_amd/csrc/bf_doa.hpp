@@ -5,13 +5,20 @@
 
 namespace bfhip {
 
-// Exact smoothed integer covariance rs [F][M][M][2] (and optionally the full
-// [F][A][A][2] covariance) of snapshots t0 .. t0+K-1.
-void launch_doa_covariance(hipStream_t stream, const uint8_t* x, size_t T, size_t F, size_t A,
-                           size_t t0, size_t K, size_t M, int* rs, int* r_full);
+// Exact smoothed integer covariance of snapshots t0 .. t0+K-1 as `parts`
+// partial sums rs [parts][F][M][M][2] over disjoint snapshot ranges (their
+// sum is the covariance), parts = doa_covariance_parts(K, max_parts), one
+// with r_full, which then also receives the full [F][A][A][2] covariance.
+// Returns parts.
+size_t doa_covariance_parts(size_t K, size_t max_parts);
+size_t launch_doa_covariance(hipStream_t stream, const uint8_t* x, size_t T, size_t F,
+                             size_t A, size_t t0, size_t K, size_t M, int* rs, int* r_full,
+                             size_t max_parts = 1);
 
-// FP64 fold, Cholesky, inverse and Bartlett/Capon scans per channel.
-void launch_doa_fold_scan(hipStream_t stream, const int* rs, double* state, bool first,
+// FP64 fold, Cholesky, inverse and Bartlett/Capon scans per channel; rs
+// holds `parts` partial covariances, summed here.
+void launch_doa_fold_scan(hipStream_t stream, const int* rs, size_t parts, double* state,
+                          bool first,
                           size_t F, size_t M, size_t K, size_t L, double alpha, double load,
                           const float* freqs, double x_step, size_t n_theta,
                           const double* sin_theta, double* capon, double* bartlett,

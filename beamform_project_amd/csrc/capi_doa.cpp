@@ -41,7 +41,8 @@ struct bf_doa {
     float* d_freqs = nullptr;
     double* d_theta = nullptr;
     double* d_sin = nullptr;
-    int* d_rs = nullptr;
+    int* d_rs = nullptr;  // [kCovParts][F][M][M][2] partial covariances
+    size_t parts = 1;     // partial covariances the last update wrote
     double* d_state = nullptr;
     double* d_capon = nullptr;
     double* d_bart = nullptr;
@@ -63,6 +64,12 @@ struct bf_doa {
         hip_check(hipStreamSynchronize(last_stream), "hipStreamSynchronize");
     }
 };
+
+// Snapshot-range parts of the covariance (workgroups per channel).  Each
+// part smooths its own partial sum, and at K = 1024, A = 64, M = 48 that
+// repeated smoothing cost more than the added parallelism saved (4 parts:
+// 49 us against 31 us for one on MI355X), so one part is used.
+constexpr size_t kCovParts = 1;
 
 extern "C" {
 
@@ -106,7 +113,7 @@ bf_status bf_doa_create(const bf_doa_config* c, bf_doa** out) {
                   "hipMemcpy theta");
         hip_check(hipMemcpy(d->d_sin, sn.data(), n * sizeof(double), hipMemcpyHostToDevice),
                   "hipMemcpy sin theta");
-        d->d_rs = dmalloc<int>(F * M * M * 2);
+        d->d_rs = dmalloc<int>(kCovParts * F * M * M * 2);
         d->d_state = dmalloc<double>(F * M * M * 2);
         hip_check(hipMemset(d->d_state, 0, F * M * M * 2 * sizeof(double)), "hipMemset state");
         d->d_capon = dmalloc<double>(F * n);
@@ -140,10 +147,10 @@ bf_status bf_doa_update(bf_doa* d, void* stream, const uint8_t* x, size_t n_time
         if (t0 + c.snapshots > n_time) invalid("the update window exceeds n_time");
         hipStream_t s = static_cast<hipStream_t>(stream);
         hip_check(hipEventRecord(d->ev[0], s), "hipEventRecord");
-        launch_doa_covariance(s, x, n_time, c.n_freq, c.n_ant, t0, c.snapshots, d->M, d->d_rs,
-                              nullptr);
+        d->parts = launch_doa_covariance(s, x, n_time, c.n_freq, c.n_ant, t0, c.snapshots,
+                                         d->M, d->d_rs, nullptr, kCovParts);
         hip_check(hipEventRecord(d->ev[1], s), "hipEventRecord");
-        launch_doa_fold_scan(s, d->d_rs, d->d_state, !d->initialised, c.n_freq, d->M,
+        launch_doa_fold_scan(s, d->d_rs, d->parts, d->d_state, !d->initialised, c.n_freq, d->M,
                              c.snapshots, d->L, c.forgetting, c.diagonal_load, d->d_freqs,
                              c.element_spacing_m, c.n_theta, d->d_sin, d->d_capon, d->d_bart,
                              d->d_ok);
@@ -213,9 +220,16 @@ bf_status bf_doa_read_covariance(bf_doa* d, int32_t* rs) {
         require_ptr(d, "doa");
         require_ptr(rs, "rs");
         d->sync();
-        hip_check(hipMemcpy(rs, d->d_rs, d->cfg.n_freq * d->M * d->M * 2 * sizeof(int),
+        const size_t n = d->cfg.n_freq * d->M * d->M * 2;
+        std::vector<int32_t> all(d->parts * n);
+        hip_check(hipMemcpy(all.data(), d->d_rs, all.size() * sizeof(int32_t),
                             hipMemcpyDeviceToHost),
                   "hipMemcpy covariance");
+        for (size_t i = 0; i < n; ++i) {
+            int32_t v = 0;
+            for (size_t p = 0; p < d->parts; ++p) v += all[p * n + i];
+            rs[i] = v;
+        }
     });
 }
 

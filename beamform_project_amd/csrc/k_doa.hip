@@ -24,6 +24,8 @@
 // v_mfma_i32_16x16x64_i8 over 64-snapshot steps; both operands are
 // "component x 16 snapshots" rows of a transposed LDS tile.  Then
 // Re R = (G_rr + G_ii)/256 and Im R = (G_ir - G_ri)/256 exactly.
+#include <algorithm>
+
 #include "bf_doa.hpp"
 #include "gauss_jordan.hpp"
 
@@ -56,14 +58,19 @@ __device__ __forceinline__ void upper_tile(int t, int n, int* I, int* J) {
     *J = i + t;
 }
 
-// One workgroup per channel; A in {32, 64} -> 2A components.  G is
+// One workgroup per (channel, part of the snapshot range); A in {32, 64}
+// -> 2A components.  Part p (blockIdx.y) covers snapshots kpart p ..
+// kpart (p + 1) - 1 of the update (never empty: the launcher sizes the
+// parts) and writes its own smoothed partial sum to rs_out + p F M M 2;
+// the consumer adds the parts (integers, so in any order).  G is
 // symmetric, so only the tiles on and above the diagonal are accumulated;
 // every entry the covariance reads for r <= c lies there, and r > c is the
 // conjugate of (c, r).
 template <int A>
 __global__ __launch_bounds__(kThreads) void covariance_kernel(
-    const uint8_t* __restrict__ x, int T, int F, int t0, int K, int M,
-    int* __restrict__ rs_out /* [F][M][M][2] */, int* __restrict__ r_full /* [F][A][A][2] or null */) {
+    const uint8_t* __restrict__ x, int T, int F, int t0, int K, int M, int kpart,
+    int* __restrict__ rs_out /* [parts][F][M][M][2] */,
+    int* __restrict__ r_full /* [F][A][A][2] or null (one part only) */) {
     constexpr int C = 2 * A;         // components
     constexpr int NTILE = C / 16;    // tile rows/cols of G
     constexpr int TILES = NTILE * (NTILE + 1) / 2;  // upper triangle
@@ -76,6 +83,12 @@ __global__ __launch_bounds__(kThreads) void covariance_kernel(
     const int per_xcd = (F + 7) / 8;
     const int f = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (f >= F) return;  // whole workgroup
+    {
+        const int kb = (int)blockIdx.y * kpart;
+        t0 += kb;
+        K = min(K - kb, kpart);
+        rs_out += (size_t)blockIdx.y * F * M * M * 2;
+    }
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -297,7 +310,8 @@ constexpr int kFoldThreads = kFoldSide * kFoldSide;   // 256
 // BS x BS Gauss-Jordan blocks: 3 for M <= 48, 4 for M <= 64.
 template <int BS>
 __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
-    const int* __restrict__ rs, double* __restrict__ state /* [F][M][M][2] */, int first,
+    const int* __restrict__ rs /* [parts][F][M][M][2] */, int parts,
+    double* __restrict__ state /* [F][M][M][2] */, int first,
     int M, int K, int L, double alpha, double load, const float* __restrict__ freqs,
     double x_step, int n_theta, const double* __restrict__ sin_theta,
     double* __restrict__ capon, double* __restrict__ bartlett, int* __restrict__ chol_ok) {
@@ -315,9 +329,15 @@ __global__ __launch_bounds__(kFoldThreads) void fold_scan_kernel(
     const double scale = 1.0 / ((double)K * (double)L);
     // R_block = Rs / (K L), folded into the carried (unloaded) state
     double* st = state + (size_t)f * MM * 2;
+    const int F = gridDim.x;
     for (int e = tid; e < MM; e += kFoldThreads) {
-        const int* p = rs + ((size_t)f * MM + e) * 2;
-        cd v{(double)p[0] * scale, (double)p[1] * scale};
+        int sre = 0, sim = 0;  // the parts' integer sums (exact)
+        for (int q = 0; q < parts; ++q) {
+            const int2 p = reinterpret_cast<const int2*>(rs)[((size_t)q * F + f) * MM + e];
+            sre += p.x;
+            sim += p.y;
+        }
+        cd v{(double)sre * scale, (double)sim * scale};
         if (!first) {
             v.re = alpha * st[2 * e] + (1.0 - alpha) * v.re;
             v.im = alpha * st[2 * e + 1] + (1.0 - alpha) * v.im;
@@ -574,13 +594,24 @@ __global__ __launch_bounds__(kThreads) void peaks_kernel(
 
 }  // namespace
 
-void launch_doa_covariance(hipStream_t stream, const uint8_t* x, size_t T, size_t F, size_t A,
-                           size_t t0, size_t K, size_t M, int* rs, int* r_full) {
+size_t doa_covariance_parts(size_t K, size_t max_parts) {
+    const size_t chunks = ceil_div(K, 64);
+    const size_t per = ceil_div(chunks, std::max<size_t>(1, std::min(max_parts, chunks)));
+    return ceil_div(chunks, per);
+}
+
+size_t launch_doa_covariance(hipStream_t stream, const uint8_t* x, size_t T, size_t F,
+                             size_t A, size_t t0, size_t K, size_t M, int* rs, int* r_full,
+                             size_t max_parts) {
     if (K == 0 || t0 + K > T) invalid("the covariance window exceeds the time axis");
-    const unsigned grid = (unsigned)(8 * ceil_div(F, 8));
+    // Parts of whole 64-snapshot chunks, none empty: part p starts at
+    // kpart p <= 64 (chunks - 1) < K.
+    const size_t parts = r_full != nullptr ? 1 : doa_covariance_parts(K, max_parts);
+    const size_t kpart = 64 * ceil_div(ceil_div(K, 64), parts);
+    const dim3 grid((unsigned)(8 * ceil_div(F, 8)), (unsigned)parts);
 #define BF_COV(A_)                                                                            \
-    hipLaunchKernelGGL(covariance_kernel<A_>, dim3(grid), dim3(kThreads), 0, stream, x,        \
-                       (int)T, (int)F, (int)t0, (int)K, (int)M, rs, r_full)
+    hipLaunchKernelGGL(covariance_kernel<A_>, grid, dim3(kThreads), 0, stream, x, (int)T,    \
+                       (int)F, (int)t0, (int)K, (int)M, (int)kpart, rs, r_full)
     switch (A) {
     case 32: BF_COV(32); break;
     case 64: BF_COV(64); break;
@@ -588,9 +619,11 @@ void launch_doa_covariance(hipStream_t stream, const uint8_t* x, size_t T, size_
     }
 #undef BF_COV
     hip_check(hipGetLastError(), "covariance_kernel launch");
+    return parts;
 }
 
-void launch_doa_fold_scan(hipStream_t stream, const int* rs, double* state, bool first,
+void launch_doa_fold_scan(hipStream_t stream, const int* rs, size_t parts, double* state,
+                          bool first,
                           size_t F, size_t M, size_t K, size_t L, double alpha, double load,
                           const float* freqs, double x_step, size_t n_theta,
                           const double* sin_theta, double* capon, double* bartlett,
@@ -601,8 +634,8 @@ void launch_doa_fold_scan(hipStream_t stream, const int* rs, double* state, bool
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
                   "hipFuncSetAttribute");
-        hipLaunchKernelGGL(kernel, dim3((unsigned)F), dim3(kFoldThreads), smem, stream, rs, state,
-                           first ? 1 : 0, (int)M, (int)K, (int)L, alpha, load, freqs, x_step,
+        hipLaunchKernelGGL(kernel, dim3((unsigned)F), dim3(kFoldThreads), smem, stream, rs,
+                           (int)parts, state, first ? 1 : 0, (int)M, (int)K, (int)L, alpha, load, freqs, x_step,
                            (int)n_theta, sin_theta, capon, bartlett, chol_ok);
     };
     if (M <= 3 * kFoldSide)

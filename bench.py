@@ -1,25 +1,28 @@
-"""Benchmark of the fixed-grid hot path on one MI355X.
+"""Benchmark of the hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c4|c5|b1|d2]
 
-Workload (default c2, the full-shard fixed grid): A=64 antennas on an 8x8
-grid at 0.3 m, F=336 channels over 400-800 MHz, T=32768 spectra, B=128 beams
-on an 8x16 (l, m) grid spanning +-0.35, one point source at a seeded
-direction with per-element SNR 3 (amplitude^2 / (2 sigma^2)) plus complex
-Gaussian noise sigma=2.5, rounded and clipped to int4.  One step = the fused
-split-FP16 MFMA beamform + |.|^2 + 10-spectrum integration (f32 output)
-followed by the int8 quantiser (codes + parameters), inputs resident in HBM.
+Default (c2, the full-shard fixed grid, BASELINE's headline): A=64 antennas on
+an 8x8 grid at 0.3 m, F=336 channels over 400-800 MHz, T=32768 spectra,
+B=128 beams on an 8x16 (l, m) grid spanning +-0.35, one point source at a
+seeded direction with per-element SNR 3 (amplitude^2 / (2 sigma^2)) plus
+complex Gaussian noise sigma=2.5, rounded and clipped to int4.  One step =
+the fused split-FP16 MFMA beamform + |.|^2 + 10-spectrum integration (f32
+output) followed by the int8 quantiser (codes + parameters), inputs
+resident in HBM.
 
-Prints one JSON line (see the keys below).  The plain FP32 fused kernel is
-timed beside it for the speedup figure; `max_rel_err_vs_fp64` compares the
-first 32 integrated windows with an FP64 (numpy complex128) evaluation of
-the same sums; `serving` runs the streaming tracker server under Poisson
-request arrivals at the stream's real-time rate (A=64, F=336, ring 2^20
-spectra, 1-64 sources, 10- or 320-spectrum windows, 0.5-5 s requests) and
-reports per-window latency percentiles and processed beam-samples/s.
-`cpu_baseline` times the reference CPU path (oracle/_ref, compiled from the
-reference's sources; the restated oracle when _ref is absent) on a bounded
-sample of the same workload.
+The default line also carries, under `legs`, the other throughput
+workloads, each with its own value, roofline and CPU baseline:
+  c1  plain fixed grid (A=32 ULA, T=1024, B=16, f32 per-spectrum output),
+  c3  dynamic tracker over the full band (F=672, T=65536, S=320, B=32),
+  c5  adaptive DOA update (K=1024 snapshots x 336 channels, M=48, 721 angles),
+and under `serving` the streaming server under Poisson request arrivals at
+the stream's real-time rate (C4).  `--workload X` prints one of them alone.
+
+CPU baselines time the reference's own CPU code (oracle/_ref, compiled from
+the reference's sources; the AVX-512 build where the host has it) on a
+bounded sample of the same workload: one warm-up run, then the median of
+five, with the host CPU model and core count in the record.
 """
 from __future__ import annotations
 
@@ -47,6 +50,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-serving", action="store_true")
+    p.add_argument("--no-legs", action="store_true",
+                   help="default line without the c1/c3/c5 legs")
     p.add_argument("--serve-seconds", type=float, default=6.0)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run three 1 GiB device-to-device copies (known bytes for "
@@ -55,10 +60,6 @@ def parse():
 
 
 def workload(name):
-    if name == "c1":
-        return dict(name="C1 plain fixed grid: A=32 ULA 0.3 m, F=336, T=1024, B=16 over "
-                         "+-60 deg, uniform int4, f32 [T][F][B] output",
-                    A=32, T=1024, B=16, S=0, kernel="FP32", ula=True)
     return dict(name="C2 full-shard fixed grid: A=64 8x8 0.3 m, F=336, T=32768, B=128 "
                      "8x16 grid, point source SNR 3 + noise sigma 2.5, S=10, f32 + int8",
                 A=64, T=32768, B=128, S=10, kernel="SPLIT_F16", ula=False)
@@ -113,9 +114,10 @@ def main():
         api.device_synchronize()
         print(json.dumps({"calibration": "3 x 1 GiB device-to-device copies"}), flush=True)
         return
-    if args.workload in ("c3", "b1", "c4", "c5", "d2"):
-        line = {"c3": bench_c3, "b1": bench_b1, "c4": bench_c4,
+    if args.workload in ("c1", "c3", "b1", "c4", "c5", "d2"):
+        line = {"c1": bench_c1, "c3": bench_c3, "b1": bench_b1, "c4": bench_c4,
                 "c5": bench_c5, "d2": bench_d2}[args.workload](args)
+        line["host"] = host_info()
         print(json.dumps(line), flush=True)
         return
     wl = workload(args.workload)
@@ -176,17 +178,21 @@ def main():
             step(k, quantize)
         stream.synchronize()
         api.device_synchronize()
-        kern_ms = []
+        kern_ms, quant_ms = [], []
         t0 = time.perf_counter()
         for _ in range(steps):
             step(k, quantize)
             kern_ms.append(ev_k0.elapsed_ms(ev_k1))  # waits for this step's kernel
+            if S and quantize:
+                quant_ms.append(ev_k1.elapsed_ms(ev_end))
         stream.synchronize()
         api.device_synchronize()
         wall = time.perf_counter() - t0
+        timed.quant_ms = float(np.mean(quant_ms)) if quant_ms else None
         return wall * 1e3 / steps, float(np.mean(kern_ms))
 
     ms_step, kern_ms = timed(kernel, args.steps, args.warmup)
+    quant_ms = timed.quant_ms
     beam_samples = T * F * B
     value = beam_samples / (ms_step * 1e-3)
 
@@ -218,13 +224,26 @@ def main():
     if S:
         err = fp64_error(x, out, w, A, B, S)
 
+    quant = None
+    if S and quant_ms:
+        qbytes = W * F * B * 4 + W * F * B + npar * 8
+        qa = qbytes / (quant_ms * 1e-3) / 1e9
+        quant = {"kernel_avg_ms": round(quant_ms, 4), "bytes_per_launch": qbytes,
+                 "achieved_GBps": round(qa, 1), "frac_of_hbm": round(qa / 8000.0, 4)}
+
     serving = None
     if not args.no_serving:
         serving = serving_run(args.serve_seconds)
 
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(wl, pos, fr, dirs, args.cpu_seconds)
+        cpu_all = cpu_baseline_all_cores(wl, pos, fr, dirs, args.cpu_seconds)
+
+    legs = {}
+    if not args.no_legs:
+        for name, fn in (("c1", bench_c1), ("c3", bench_c3), ("c5", bench_c5)):
+            legs[name] = fn(args)
 
     line = {
         "metric": "beam-samples/s (T·F·B)",
@@ -247,12 +266,16 @@ def main():
         "p99_window_latency_ms": serving["latency_p99_ms"] if serving else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
+        "quantizer": quant,
         "serving": serving,
         "fp32_fused_kernel": {"ms_per_step": round(fp32_ms, 4),
                               "kernel_avg_ms": round(fp32_kern, 4),
                               "beam_samples_per_s": round(beam_samples / (fp32_ms * 1e-3), 1),
                               "split_f16_kernel_avg_ms": round(split_kern, 4),
                               "split_over_fp32_kernel": round(fp32_kern / split_kern, 2)},
+        "legs": legs,
+        "host": host_info(),
     }
     print(json.dumps(line), flush=True)
 
@@ -264,6 +287,29 @@ def common_line(args, metric, value, unit, ms, dtype, config, **extra):
             "data": "synthetic (device-generated, seeded)", "config": config}
     line.update(extra)
     return line
+
+
+def host_info():
+    """The host the CPU baselines ran on."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker/baseline library, never the product path
+    h = O.host_cpu()
+    h["omp_num_threads"] = int(os.environ.get("OMP_NUM_THREADS", h["nproc"]))
+    h["avx512"] = O.host_has_avx512()
+    return h
+
+
+def median_of_five(run, size0, step, seconds, hi=None):
+    """CPU timing protocol of the reference's benchmarks
+    (benchmarks/benchmark_beam_tracker_opt_v2.cpp:16-21,59-60): the sample
+    size is calibrated so one run takes about seconds/6, then one warm-up
+    run and the median of five timed runs.  Returns (size, median_s)."""
+    dt = run(size0)
+    size = max(size0, int(size0 * (seconds / 6.0) / max(dt, 1e-6)) // step * step)
+    if hi is not None:
+        size = min(size, hi)
+    run(size)
+    return size, float(np.median([run(size) for _ in range(5)]))
 
 
 def timed_launches(stream, launch, steps, warmup):
@@ -373,15 +419,17 @@ def bench_c3(args):
 def tracker_cpu_baseline(seconds, B, F_):
     """The reference's OpenMP CPU tracker (beam_tracker_opt_v2, single beam,
     F=336 as it requires) on all host cores over a bounded T sample, scaled
-    to beam-samples/s of B beams over F_ channels."""
+    to beam-samples/s of B beams over F_ channels (the tracker runs once per
+    source and per 336-channel shard)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the checker/baseline library, never the product path
-    R = O.ref_lib()
+    R = O.ref_lib(native=True)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     A, F0, S = 64, 336, 320
+    x_all = O.make_uniform_int4(S * 4 * F0 * A, 9).reshape(S * 4, F0, A)
 
     def run(T):
-        x = O.make_uniform_int4(T * F0 * A, 9)
+        x = np.ascontiguousarray(np.resize(x_all, (T, F0, A)))
         out = np.zeros((T, F0, 1), np.float32)
         t0 = time.perf_counter()
         if R is not None:
@@ -390,14 +438,13 @@ def tracker_cpu_baseline(seconds, B, F_):
             O.beam_tracker(x, T, F0, A, 0.0, 0.0, 1e-5, 0.0, S, threads=True)
         return time.perf_counter() - t0
 
-    dt = run(S)
-    T = max(S, int(S * seconds / 3 / max(dt, 1e-6)) // S * S)
-    med = float(np.median([run(T) for _ in range(3)]))
+    T, med = median_of_five(run, S, S, seconds)
     return {"value": round(T * F0 / med, 1), "unit": "beam-samples/s", "cores": threads,
             "kind": "reference" if R is not None else "port",
             "sample": f"beam_tracker_opt_v2 (OpenMP), one beam, T={T} x F={F0}, A={A}, "
-                      f"S={S}; median of 3 runs of {med:.2f} s; the C3 job is B={B} such "
-                      f"beams over F={F_} channels"}
+                      f"S={S}; 1 warm-up, median of 5 runs of {med:.3f} s; the C3 job is "
+                      f"B={B} such beams over F={F_} channels, so its CPU time is "
+                      f"{B * F_ // F0} times this per beam-sample"}
 
 
 def bench_b1(args):
@@ -586,29 +633,46 @@ def opt_tracker_cpu_baseline(seconds):
 
 
 def doa_cpu_baseline(seconds, A, K, M, NTH):
-    """The FP64 C restatement of the DOA update (oracle, single thread) on a
-    bounded subset of channels, scaled to snapshot-channels/s."""
+    """The reference's FP32 DOA primitives (src/cpu_opt_beam_tracker.cpp,
+    composed per update by oracle/ref_doa_shim.cpp: covariance with
+    sub-array smoothing, diagonal load, fold, Cholesky, Bartlett and Capon
+    over the angle grid, OpenMP over channels as in run_into) on all host
+    cores, over a bounded subset of the channels, scaled to
+    snapshot-channels/s.  The FP64 C restatement stands in where
+    oracle/_ref was not built."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the checker/baseline library, never the product path
+    lib = O.ref_doa_lib(native=True)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    theta = np.linspace(-np.pi / 2, np.pi / 2, NTH)
+    xpos = (0.3 * np.arange(A)).astype(np.float32)
 
     def run(Fs):
-        x = O.make_uniform_int4(K * Fs * A, 3).reshape(K, Fs, A)
+        x = O.make_uniform_int4(K * Fs * A, 3)
         fr = np.linspace(400e6, 800e6, Fs).astype(np.float32)
-        xpos = (0.3 * np.arange(A)).astype(np.float32)
-        theta = np.linspace(-np.pi / 2, np.pi / 2, NTH)
-        state = np.zeros((Fs, M, M, 2), np.float64)
-        load = np.zeros(Fs, np.float64)
         t0 = time.perf_counter()
-        O.doa_update(x, K, Fs, A, 0, K, M, 0.98, 1e-3, True, fr, xpos, theta, state, load)
+        if lib is not None:
+            Rb = np.zeros(Fs * M * M * 2, np.float32)
+            Rc = np.zeros(Fs * M * M * 2, np.float32)
+            pb = np.zeros(NTH, np.float32)
+            pc = np.zeros(NTH, np.float32)
+            assert lib.ref_doa_update_f32(x, K, Fs, A, 0, K, M, 0.98, 1e-3, 1, fr, xpos, NTH,
+                                          theta, Rb, Rc, pb, pc) == 0
+        else:
+            state = np.zeros((Fs, M, M, 2), np.float64)
+            load = np.zeros(Fs, np.float64)
+            O.doa_update(x, K, Fs, A, 0, K, M, 0.98, 1e-3, True, fr, xpos, theta, state, load)
         return time.perf_counter() - t0
 
-    dt = run(1)
-    Fs = int(max(1, min(336, seconds / 3 / max(dt, 1e-6))))
-    med = float(np.median([run(Fs) for _ in range(3)]))
-    return {"value": round(K * Fs / med, 1), "unit": "snapshot-channels/s", "cores": 1,
-            "kind": "port",
-            "sample": f"one update over {Fs} channels (K={K}, A={A}, M={M}, {NTH} angles) of "
-                      f"the FP64 C restatement; median of 3 runs of {med:.2f} s"}
+    step = max(1, threads)
+    Fs, med = median_of_five(run, step, step, seconds, hi=336)
+    return {"value": round(K * Fs / med, 1), "unit": "snapshot-channels/s",
+            "cores": threads if lib is not None else 1,
+            "kind": "reference" if lib is not None else "port",
+            "sample": f"one update (K={K}, A={A}, M={M} with the reference's smoothing "
+                      f"and forward-backward step, {NTH} angles, Bartlett + Capon) over "
+                      f"{Fs} channels through the reference's FP32 DOA primitives; 1 "
+                      f"warm-up, median of 5 runs of {med:.3f} s"}
 
 
 def fp64_error(x, out, w, A, B, S, n_windows=32):
@@ -698,20 +762,23 @@ def read_profile_value(name, key):
     return vals.get(key)
 
 
-def cpu_baseline(wl, pos, fr, dirs, seconds):
-    """The reference's single-threaded CPU fixed-grid path on a bounded
-    prefix of the workload (T_sample spectra, all channels and beams),
-    scaled to beam-samples/s."""
+def cpu_baseline(wl, pos, fr, dirs, seconds, per_spectrum=False):
+    """The reference's single-threaded CPU fixed-grid path
+    (cpu_beamform_packed_intensity_into, then cpu_integrate_intensity and
+    cpu_quantize_integrated_intensity when the workload integrates) on a
+    bounded prefix of the workload (T_sample spectra, all channels and
+    beams), scaled to beam-samples/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the checker/baseline library, never the product path
 
-    A, B, S = wl["A"], wl["B"], wl["S"]
+    A, B, S = wl["A"], wl["B"], (0 if per_spectrum else wl["S"])
     w = weights_from(pos, fr, dirs)
-    R = O.ref_lib()
+    R = O.ref_lib(native=True)
     kind = "reference" if R is not None else "port"
+    x_all = O.make_uniform_int4(1024 * F * A, 5).reshape(1024, F, A)
 
     def run(T):
-        x = O.make_uniform_int4(T * F * A, 5).reshape(T, F, A)
+        x = np.ascontiguousarray(np.resize(x_all, (T, F, A)))
         out = np.zeros((T, F, B), np.float32)
         t0 = time.perf_counter()
         if R is not None:
@@ -726,25 +793,116 @@ def cpu_baseline(wl, pos, fr, dirs, seconds):
         else:
             o = O.beamform(x, w, T, F, A, B)
             if S:
-                Wn = (T + S - 1) // S
-                O.quantize(O.integrate(o, T, F, B, S), Wn, F, B)
+                O.quantize(O.integrate(o, T, F, B, S), (T + S - 1) // S, F, B)
         return time.perf_counter() - t0
 
-    t_cal = 10 if S else 8
-    dt = run(t_cal)
-    T = max(t_cal, int(t_cal * seconds / max(dt, 1e-6)))
-    if S:
-        T = max(S, (T // S) * S)
-    times = [run(T) for _ in range(3)]
-    med = float(np.median(times))
+    step = S if S else 1
+    T, med = median_of_five(run, max(step, 10), step, seconds, hi=wl["T"])
+    stages = ("cpu_beamform_packed_intensity_into"
+              + (" + cpu_integrate_intensity + cpu_quantize_integrated_intensity" if S else ""))
     return {"value": round(T * F * B / med, 1), "unit": "beam-samples/s", "cores": 1,
             "kind": kind,
-            "sample": f"T={T} spectra x F={F} x B={B} (A={A}) through "
-                      f"cpu_beamform_packed_intensity_into"
-                      + (" + cpu_integrate_intensity + cpu_quantize_integrated_intensity"
-                         if S else "")
-                      + f"; median of 3 runs, {med:.2f} s each; single thread as in the "
-                        "reference"}
+            "sample": f"T={T} spectra x F={F} x B={B} (A={A}) through {stages}; 1 warm-up, "
+                      f"median of 5 runs of {med:.3f} s; single thread as in the reference"}
+
+
+def cpu_baseline_all_cores(wl, pos, fr, dirs, seconds):
+    """Not the reference: its fixed-grid loop parallelised naively over
+    (t, f) on all host cores (the oracle's OpenMP restatement, same
+    arithmetic), then integration and quantisation, as an upper bound for a
+    CPU implementation of the same loop."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker/baseline library, never the product path
+
+    A, B, S = wl["A"], wl["B"], wl["S"]
+    w = weights_from(pos, fr, dirs)
+    x_all = O.make_uniform_int4(1024 * F * A, 5).reshape(1024, F, A)
+
+    def run(T):
+        x = np.ascontiguousarray(np.resize(x_all, (T, F, A)))
+        t0 = time.perf_counter()
+        o = O.beamform(x, w, T, F, A, B, threads=True)
+        if S:
+            O.quantize(O.integrate(o, T, F, B, S), (T + S - 1) // S, F, B)
+        return time.perf_counter() - t0
+
+    step = S if S else 1
+    T, med = median_of_five(run, max(step, 10), step, seconds)
+    return {"value": round(T * F * B / med, 1), "unit": "beam-samples/s",
+            "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+            "kind": "port",
+            "sample": f"NOT the reference (which is single-threaded): its loop over (t, f) "
+                      f"in OpenMP, T={T} x F={F} x B={B} (A={A}), then integration and "
+                      f"quantisation; 1 warm-up, median of 5 runs of {med:.3f} s"}
+
+
+def bench_c1(args):
+    """Plain fixed grid: A=32 ULA at 0.3 m, F=336 over 400-800 MHz,
+    T=1024, B=16 beams over +-60 deg, uniform int4, f32 [T][F][B] output."""
+    from beamform_project_amd import api
+    A, T, B = 32, 1024, 16
+    wl = dict(A=A, T=T, B=B, S=0, ula=True)
+    pos, fr, dirs = host_geometry(wl)
+    w = weights_from(pos, fr, dirs)
+    stream = api.Stream()
+    x = api.DeviceArray.empty((T, F, A), np.uint8)
+    api.generate_uniform_int4(stream, x, x.nbytes, 1234)
+    dw = api.DeviceArray.from_numpy(w)
+    out = api.DeviceArray.empty((T, F, B), np.float32)
+    stream.synchronize()
+    dims = api.Dimensions(T, F, A, B)
+    steps = max(args.steps, 50)
+    res = {}
+    for name in ("SPLIT_F16", "F16", "FP32"):
+        kern = api.Kernel[name]
+
+        def launch():
+            api.launch_packed_beamformer(kern, stream, x, dw, out, dims)
+        ms, kms = timed_launches(stream, launch, steps, args.warmup)
+        err = fp64_error_spectra(x, out, w, T, A, B)
+        res[name] = {"ms_per_step": round(ms, 4), "kernel_avg_ms": round(kms, 4),
+                     "beam_samples_per_s": round(T * F * B / (ms * 1e-3), 1),
+                     "max_rel_err_vs_fp64": err}
+    split = res["SPLIT_F16"]
+    bytes_alg = T * F * A + B * F * A * 8 + T * F * B * 4
+    ach = bytes_alg / (split["kernel_avg_ms"] * 1e-3) / 1e9
+    flops = 8 * T * F * B * A + 3 * T * F * B
+    fp32_ach = flops / (res["FP32"]["kernel_avg_ms"] * 1e-3) / 1e12
+    cpu = None if args.no_cpu_baseline else cpu_baseline(wl, pos, fr, dirs, args.cpu_seconds,
+                                                         per_spectrum=True)
+    return common_line(
+        args, "beam-samples/s (T·F·B)", T * F * B / (split["ms_per_step"] * 1e-3),
+        "beam-samples/s", split["ms_per_step"], "f16 split hi+lo, f32 accumulate",
+        {"workload": "C1 plain fixed grid: A=32 ULA 0.3 m, F=336 (400-800 MHz), T=1024, "
+                     "B=16 over +-60 deg, uniform int4, f32 [T][F][B] output",
+         "n_ant": A, "n_freq": F, "n_time": T, "n_beams": B, "integration_spectra": 0,
+         "kernel": "SPLIT_F16"},
+        max_rel_err_vs_fp64=split["max_rel_err_vs_fp64"],
+        roofline={"bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
+                  "frac": round(ach / 8000.0, 4),
+                  "traffic": read_profile_value("traffic_c1_split_f16.txt",
+                                                "hbm_bytes_per_launch"),
+                  "kernel_avg_ms": split["kernel_avg_ms"], "algorithmic_bytes": bytes_alg},
+        fp32_roofline={"bound": "valu-fp32", "achieved": round(fp32_ach, 2), "peak": 157.3,
+                       "unit": "TFLOP/s", "frac": round(fp32_ach / 157.3, 4)},
+        cpu_baseline=cpu, paths=res)
+
+
+def fp64_error_spectra(x, out, w, T, A, B):
+    """Floored max relative error of a per-spectrum output [T][F][B] against
+    complex128 sums with the same f32 weights; floor = A * E|x|^2."""
+    from beamform_project_amd import _capi
+    xb = np.empty((T, F, A), np.uint8)
+    _capi.call("bf_memcpy_dtoh", xb.ctypes.data, x.ptr, xb.nbytes)
+    got = out.to_numpy()
+    re = ((xb & 0xF).astype(np.int16) ^ 8) - 8
+    im = ((xb >> 4).astype(np.int16) ^ 8) - 8
+    xc = re.astype(np.float64) + 1j * im.astype(np.float64)
+    wc = w[..., 0].astype(np.float64) + 1j * w[..., 1].astype(np.float64)  # [B][F][A]
+    v = np.matmul(xc.transpose(1, 0, 2), wc.transpose(1, 2, 0))  # [F][T][B]
+    p = (v.real ** 2 + v.imag ** 2).transpose(1, 0, 2)
+    floor = A * float(np.mean(np.abs(xc) ** 2))
+    return float(np.max(np.abs(got - p) / np.maximum(p, floor)))
 
 
 if __name__ == "__main__":

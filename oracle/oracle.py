@@ -329,11 +329,67 @@ def doa_peaks(P, theta, max_peaks):
 
 
 # ------------------------------------------------------- reference (_ref)
-def ref_lib():
-    """The reference CPU core compiled from its own sources, or None."""
-    if not os.path.exists(REF_PATH):
+def host_has_avx512() -> bool:
+    """True where /proc/cpuinfo lists the AVX-512 subsets x86-64-v4 needs."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("flags"):
+                    have = set(ln.split(":", 1)[1].split())
+                    return {"avx512f", "avx512bw", "avx512cd", "avx512dq",
+                            "avx512vl"} <= have
+    except OSError:
+        pass
+    return False
+
+
+def host_cpu() -> dict:
+    """CPU model, logical CPUs usable by this process, and sockets."""
+    model, sockets = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name") and model == "unknown":
+                    model = ln.split(":", 1)[1].strip()
+                elif ln.startswith("physical id"):
+                    sockets.add(ln.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return {"model": model, "nproc": len(os.sched_getaffinity(0)),
+            "sockets": max(1, len(sockets))}
+
+
+def _ref_path(name: str, native: bool) -> str:
+    """oracle/_ref/<name>, or its x86-64-v4 build when `native` and the host
+    has AVX-512 (the nearest to the reference perf core's -march=native)."""
+    if native and host_has_avx512():
+        p = os.path.join(HERE, "_ref", "v4", name)
+        if os.path.exists(p):
+            return p
+    return os.path.join(HERE, "_ref", name)
+
+
+def ref_doa_lib(native: bool = True):
+    """The reference's FP32 DOA primitives (ref_doa_shim.cpp), or None."""
+    path = _ref_path("libbfref_doa.so", native)
+    if not os.path.exists(path):
         return None
-    lib = C.CDLL(REF_PATH)
+    lib = C.CDLL(path)
+    fn = lib.ref_doa_update_f32
+    fn.restype = C.c_int
+    fn.argtypes = [_u8p, _sz, _sz, _sz, _sz, _sz, _sz, C.c_float, C.c_float, C.c_int, _f32p,
+                   _f32p, _sz, _f64p, _f32p, _f32p, _f32p, _f32p]
+    return lib
+
+
+def ref_lib(native: bool = False):
+    """The reference CPU core compiled from its own sources, or None.  The
+    tests load the portable x86-64-v3 build; `native` (the CPU baselines)
+    takes the AVX-512 build where the host supports it."""
+    path = _ref_path("libbfref.so", native)
+    if not os.path.exists(path):
+        return None
+    lib = C.CDLL(path)
     sigs = {
         "ref_default_positions": [_sz, C.c_float, _f32p],
         "ref_channelized_frequencies": [_sz, C.c_float, C.c_float, _f32p],

@@ -471,6 +471,41 @@ def test_doa_update_recursion_matches_reference_definition():
         assert worst > 1e-6, variant
 
 
+def test_reference_doa_primitives_agree_with_the_fp64_update():
+    """The C5 CPU baseline (the reference's FP32 DOA primitives from
+    src/cpu_opt_beam_tracker.cpp, composed by oracle/ref_doa_shim.cpp)
+    computes the same broadband spectra as the FP64 restatement over three
+    updates with forgetting and a diagonal load, to FP32 accuracy.  Without
+    smoothing (M = A) the reference's covariance has no forward-backward
+    step, so both sides define the same quantity."""
+    lib = O.ref_doa_lib(native=False)
+    if lib is None:
+        pytest.skip("oracle/_ref was not built (no reference tree)")
+    A = M = 32
+    K, Fs, n_up = 256, 4, 3
+    alpha, load = 0.98, 1e-3
+    x = O.make_uniform_int4(n_up * K * Fs * A, 17)
+    T = n_up * K
+    fr = np.linspace(400e6, 800e6, Fs).astype(np.float32)
+    xpos = (0.3 * np.arange(A)).astype(np.float32)
+    theta = np.linspace(-np.pi / 2, np.pi / 2, 181)
+    R64 = np.zeros((Fs, M, M, 2), np.float64)
+    L64 = np.zeros(Fs, np.float64)
+    Rb = np.zeros(Fs * M * M * 2, np.float32)
+    Rc = np.zeros(Fs * M * M * 2, np.float32)
+    for u in range(n_up):
+        cp, bt, ok = O.doa_update(x, T, Fs, A, u * K, K, M, alpha, load, u == 0, fr, xpos,
+                                  theta, R64, L64)
+        pb = np.zeros(len(theta), np.float32)
+        pc = np.zeros(len(theta), np.float32)
+        assert lib.ref_doa_update_f32(x, T, Fs, A, u * K, K, M, alpha, load, int(u == 0), fr,
+                                      xpos, len(theta), theta, Rb, Rc, pb, pc) == 0
+        assert ok.all()
+        eb = np.max(np.abs(pb - bt.sum(0)) / bt.sum(0).max())
+        ec = np.max(np.abs(pc - cp.sum(0)) / np.abs(cp.sum(0)))
+        assert eb < 1e-5 and ec < 1e-3, (u, eb, ec)
+
+
 def test_opt_tracker_restatement_matches_reference_fixtures():
     """The numpy restatement of the adaptive search (oracle.opt_tracker_search)
     against the reference's CpuOptBeamTracker directions, with the
