@@ -313,21 +313,22 @@ def median_of_five(run, size0, step, seconds, hi=None):
 
 
 def timed_launches(stream, launch, steps, warmup):
-    """Average wall ms per launch and average HIP-event ms of the launch."""
+    """Average wall ms per launch, and the average HIP-event ms per launch:
+    the events bracket `steps` back-to-back launches on `stream`, so a short
+    kernel's figure is its own duration, not the host's launch gap."""
     from beamform_project_amd import api
     e0, e1 = api.Event(), api.Event()
     for _ in range(warmup):
         launch()
     stream.synchronize()
-    kern = []
     t0 = time.perf_counter()
+    e0.record(stream)
     for _ in range(steps):
-        e0.record(stream)
         launch()
-        e1.record(stream)
-        kern.append(e0.elapsed_ms(e1))
+    e1.record(stream)
     stream.synchronize()
-    return (time.perf_counter() - t0) * 1e3 / steps, float(np.mean(kern))
+    wall = time.perf_counter() - t0
+    return wall * 1e3 / steps, e0.elapsed_ms(e1) / steps
 
 
 def tracker_fp64_error(x_dev, out, pos, fr, dirs, T0, F_, A, B, S):
