@@ -113,6 +113,12 @@ SIGNATURES = {
     "bf_workspace_run_quantized_integrated_pipeline": [_vp, _vp, _sz, _vp, _sz, _vp, _sz,
                                                        _pd, _pt],
     "bf_workspace_run_device_frame": [_vp, _pf, _pf, _pf, _pf, _pt],
+    "bf_offline_frame_runner_create": [_pd, C.c_int, _sz, C.c_int, C.POINTER(_vp)],
+    "bf_offline_frame_runner_destroy": [_vp],
+    "bf_offline_frame_runner_config": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)],
+    "bf_offline_frame_runner_run": [_vp, _vp, _sz, _vp, _sz, _pd, C.c_uint64,
+                                    C.POINTER(Shard), _vp, _sz, _vp, _sz, _vp, _sz, _pd, _pt],
     "bf_beamform_packed_intensity": [_vp, _sz, _vp, _sz, _pd, C.c_int, _vp, _sz, _pt],
     "bf_beamform_packed_integrated_intensity": [_vp, _sz, _vp, _sz, _pd, _sz, C.c_int, _vp,
                                                 _sz, _pt],

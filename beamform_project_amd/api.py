@@ -403,6 +403,72 @@ class BeamformerWorkspace:
         return t.as_dict()
 
 
+class OfflineFrameRunner:
+    """Reference include/beamformer/cuda_offline_runner.hpp:17-52
+    (CudaOfflineFrameRunner): one offline frame from host arrays through
+    temporary device copies and the workspace's device-frame path."""
+
+    def __init__(self, capacity: Dimensions, kernel: Kernel = Kernel.DIRECT,
+                 temporal_integration: int | None = None, output: Output = Output.FLOAT32):
+        self._h = None
+        h = C.c_void_p()
+        call("bf_offline_frame_runner_create", C.byref(capacity.c()), int(kernel),
+             int(temporal_integration or 0), int(output), C.byref(h))
+        self._h = h.value
+        self._kernel, self._output = Kernel(kernel), Output(output)
+        self._spectra = int(temporal_integration or 0)
+
+    def kernel(self) -> Kernel:
+        return self._kernel
+
+    def output(self) -> Output:
+        return self._output
+
+    def has_temporal_integration(self) -> bool:
+        return self._spectra != 0
+
+    def run(self, packed, weights, dims: Dimensions, frame_id: int, shard: Shard) -> dict:
+        """Returns {"output_dims", "timings", "float32_output"} or, for int8
+        runners, {"output_dims", "timings", "codes", "parameters"}."""
+        packed = np.ascontiguousarray(packed, np.uint8)
+        weights = np.ascontiguousarray(weights, np.float32)
+        rows = integrated_time_count(dims.n_time, self._spectra) if self._spectra else dims.n_time
+        shape = (rows, dims.n_freq, dims.n_beams)
+        od, t = Dims(), Timings()
+        res = {}
+        if self._output == Output.INT8:
+            codes = np.zeros(shape, np.int8)
+            npar = quantization_parameter_count(Dimensions(rows, dims.n_freq, dims.n_ant,
+                                                           dims.n_beams))
+            par = np.zeros((npar, 2), np.float32)
+            call("bf_offline_frame_runner_run", self._h, packed.ctypes.data, packed.size,
+                 weights.ctypes.data, weights.size // 2, C.byref(dims.c()), frame_id,
+                 C.byref(shard), None, 0, codes.ctypes.data, codes.size, par.ctypes.data,
+                 npar, C.byref(od), C.byref(t))
+            res.update(codes=codes, parameters=par)
+        else:
+            out = np.zeros(shape, np.float32)
+            call("bf_offline_frame_runner_run", self._h, packed.ctypes.data, packed.size,
+                 weights.ctypes.data, weights.size // 2, C.byref(dims.c()), frame_id,
+                 C.byref(shard), out.ctypes.data, out.size, None, 0, None, 0, C.byref(od),
+                 C.byref(t))
+            res.update(float32_output=out)
+        res["output_dims"] = Dimensions(od.n_time, od.n_freq, od.n_ant, od.n_beams)
+        res["timings"] = t.as_dict()
+        return res
+
+    def close(self) -> None:
+        if self._h:
+            LIB.bf_offline_frame_runner_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # --------------------------------------------------------------- frame views
 def default_shard_descriptors():
     """Reference include/beamformer/config.hpp:65-72."""

@@ -121,9 +121,9 @@ void validate_integration(int kernel, size_t S) {
     if (S == 0) invalid("integration_spectra must be positive");
     if (is_reference_kernel(kernel)) {
         if (S != 10 && S != 320)
-            invalid("CUDA temporal integration supports only 10 or 320 spectra");
+            invalid("the Direct and Tiled selectors integrate over 10 or 320 spectra only");
         if (kernel == BF_KERNEL_DIRECT && S != 320)
-            invalid("the Direct CUDA kernel supports only 320-spectrum temporal integration");
+            invalid("the Direct selector integrates over 320 spectra only");
     }
     if ((kernel == BF_KERNEL_SPLIT_F16 || kernel == BF_KERNEL_F16) && !(S == 10 || S % 4 == 0))
         invalid("the MFMA kernels support integration over 10 spectra or a multiple of 4");
@@ -184,7 +184,7 @@ size_t dtype_size(int t) {
     case BF_DTYPE_SIGNED_INT8: return 1;
     case BF_DTYPE_COMPLEX_FLOAT32: return 8;
     case BF_DTYPE_FLOAT32: return 4;
-    default: invalid("unknown CUDA data type");
+    default: invalid("unknown frame element type");
     }
 }
 
@@ -199,15 +199,15 @@ bool same_shard(const bf_shard& a, const bf_shard& b) {
 
 // Reference include/beamformer/cuda_frame.hpp:62-149.
 size_t element_count(const bf_frame_view& v) {
-    if (v.rank == 0 || v.rank > 4) invalid("CUDA frame rank must be between 1 and 4");
+    if (v.rank == 0 || v.rank > 4) invalid("a frame view has rank 1 to 4");
     size_t n = 1;
     for (size_t i = 0; i < v.rank; ++i) {
         if (v.shape[i] == 0 || n > std::numeric_limits<size_t>::max() / v.shape[i])
-            invalid("CUDA frame shape is empty or overflows size_t");
+            invalid("a frame view's shape has a zero extent or its element count overflows");
         n *= v.shape[i];
     }
     for (size_t i = v.rank; i < 4; ++i)
-        if (v.shape[i] != 0) invalid("unused CUDA frame shape axes must be zero");
+        if (v.shape[i] != 0) invalid("a frame view's shape entries past its rank must be 0");
     return n;
 }
 
@@ -215,14 +215,14 @@ size_t minimum_storage(const bf_frame_view& v) {
     element_count(v);
     size_t last = dtype_size(v.type);
     for (size_t i = 0; i < v.rank; ++i) {
-        if (v.strides_bytes[i] == 0) invalid("CUDA buffer strides must be positive");
+        if (v.strides_bytes[i] == 0) invalid("a frame view's strides must be nonzero");
         const size_t extent = v.shape[i] - 1;
         if (extent > (std::numeric_limits<size_t>::max() - last) / v.strides_bytes[i])
-            invalid("CUDA buffer storage size overflows size_t");
+            invalid("a frame view's byte extent overflows");
         last += extent * v.strides_bytes[i];
     }
     for (size_t i = v.rank; i < 4; ++i)
-        if (v.strides_bytes[i] != 0) invalid("unused CUDA buffer strides must be zero");
+        if (v.strides_bytes[i] != 0) invalid("a frame view's strides past its rank must be 0");
     return last;
 }
 
@@ -238,15 +238,15 @@ bool is_contiguous(const bf_frame_view& v) {
 
 void validate_frame(const bf_frame_view* v, const char* label) {
     if (v == nullptr) invalid(std::string(label) + " frame must not be null");
-    if (v->device_data == nullptr) invalid("CUDA buffer device_data must not be null");
+    if (v->device_data == nullptr) invalid("a frame view's device pointer is null");
     if (v->bytes < minimum_storage(*v))
-        invalid("CUDA buffer bytes are smaller than its shape and strides");
+        invalid("a frame view's byte count is below what its shape and strides address");
     validate_shard(v->shard);
 }
 
 void validate_frame_shape(const bf_frame_view& v, int type, size_t rank,
                           const std::array<size_t, 4>& shape, const char* label) {
-    if (!is_contiguous(v)) invalid("CUDA frame must use contiguous storage");
+    if (!is_contiguous(v)) invalid("the frame view must be densely packed (row-major strides)");
     bool same = v.type == type && v.rank == rank;
     for (int i = 0; i < 4; ++i) same = same && v.shape[i] == shape[i];
     if (!same) invalid(std::string(label) + " shape or dtype does not match");
@@ -475,7 +475,7 @@ struct bf_workspace {
         validate_dimensions(d);
         if (d.n_freq != capacity.n_freq || d.n_ant != capacity.n_ant ||
             d.n_time > capacity.n_time || d.n_beams > capacity.n_beams)
-            invalid("dimensions exceed CUDA workspace capacity");
+            invalid("the request exceeds the workspace capacity (n_time and n_beams may shrink; n_freq and n_ant are fixed)");
     }
     // Reference src/cuda_beamformer.cu:581-605.
     void require_loaded(const bf_dims& d) const {
@@ -489,12 +489,12 @@ struct bf_workspace {
     }
     void require_integration() const {
         if (spectra == 0 || integrated.empty())
-            fail(BF_ERR_LOGIC, "CUDA workspace was not constructed with temporal integration");
+            fail(BF_ERR_LOGIC, "this workspace was created without temporal integration");
     }
     void require_quantization() const {
         require_integration();
         if (output != BF_OUTPUT_INT8 || codes.empty() || params.empty())
-            fail(BF_ERR_LOGIC, "CUDA workspace was not constructed with int8 quantization");
+            fail(BF_ERR_LOGIC, "this workspace was created without int8 output");
     }
     float* intensity_buffer() {
         if (intensity.empty())
@@ -881,6 +881,185 @@ bf_status bf_workspace_run_device_frame(bf_workspace* ws, const bf_frame_view* v
                 timings->quantization_ms = elapsed(ws->kernel_end, ws->quantization_end);
             timings->device_to_device_ms = elapsed(*producer_end, ws->result_end);
         }
+    });
+}
+
+// ------------------------------------------------- offline frame runner
+}  // extern "C"
+
+// One offline frame from host buffers: temporary device copies of the
+// caller's voltages and weights, the workspace's device-frame path, and the
+// result copied back (src/cuda_offline_runner.cu:130-228).
+struct bf_offline_frame_runner {
+    bf_workspace* ws = nullptr;
+    bf_dims capacity{};
+    int kernel = BF_KERNEL_DIRECT;
+    size_t spectra = 0;
+    int output = BF_OUTPUT_FLOAT32;
+    ~bf_offline_frame_runner() {
+        if (ws) (void)bf_workspace_destroy(ws);
+    }
+};
+
+namespace {
+
+bf_frame_view contiguous_view(void* data, int type, size_t rank,
+                              const std::array<size_t, 4>& shape, uint64_t frame_id,
+                              const bf_shard& shard) {
+    bf_frame_view v{};
+    v.device_data = data;
+    v.type = type;
+    v.rank = rank;
+    size_t stride = dtype_size(type);
+    for (size_t i = rank; i-- > 0;) {
+        v.shape[i] = shape[i];
+        v.strides_bytes[i] = stride;
+        stride *= shape[i];
+    }
+    v.bytes = stride;
+    v.frame_id = frame_id;
+    v.shard = shard;
+    return v;
+}
+
+void rethrow_status(bf_status st) {
+    if (st != BF_OK) fail(st, std::string(bf_last_error_message()));
+}
+
+}  // namespace
+
+extern "C" {
+
+bf_status bf_offline_frame_runner_create(const bf_dims* capacity, int kernel,
+                                         size_t integration_spectra, int output,
+                                         bf_offline_frame_runner** out) {
+    return guarded([&] {
+        require_ptr(out, "out");
+        *out = nullptr;
+        auto r = std::make_unique<bf_offline_frame_runner>();
+        rethrow_status(bf_workspace_create(capacity, kernel, integration_spectra, output, &r->ws));
+        r->capacity = *capacity;
+        r->kernel = kernel;
+        r->spectra = integration_spectra;
+        r->output = output;
+        *out = r.release();
+    });
+}
+
+bf_status bf_offline_frame_runner_destroy(bf_offline_frame_runner* r) {
+    return guarded([&] { delete r; });
+}
+
+bf_status bf_offline_frame_runner_config(const bf_offline_frame_runner* r, int* kernel,
+                                         int* output, int* has_temporal_integration) {
+    return guarded([&] {
+        require_ptr(r, "offline frame runner");
+        if (kernel) *kernel = r->kernel;
+        if (output) *output = r->output;
+        if (has_temporal_integration) *has_temporal_integration = r->spectra != 0;
+    });
+}
+
+bf_status bf_offline_frame_runner_run(bf_offline_frame_runner* r, const uint8_t* packed,
+                                      size_t packed_bytes, const float* weights,
+                                      size_t weight_count, const bf_dims* dims,
+                                      uint64_t frame_id, const bf_shard* shard,
+                                      float* float_output, size_t float_count,
+                                      int8_t* codes, size_t code_count, float* parameters,
+                                      size_t parameter_count, bf_dims* output_dims,
+                                      bf_timings* timings) {
+    return guarded([&] {
+        require_ptr(r, "offline frame runner");
+        const bf_dims& d = deref_dims(dims);
+        validate_dimensions(d);
+        require_ptr(shard, "shard");
+        rethrow_status(bf_validate_shard_descriptor(shard));
+        const bf_dims& cap = r->capacity;
+        if (d.n_freq != cap.n_freq || d.n_ant != cap.n_ant || d.n_time > cap.n_time ||
+            d.n_beams > cap.n_beams)
+            invalid("the frame's dimensions exceed the runner's capacity");
+        if (packed == nullptr || packed_bytes != d.n_time * d.n_freq * d.n_ant)
+            invalid("packed voltage byte count does not match the frame's dimensions");
+        const bool tiled = r->kernel == BF_KERNEL_TILED;
+        const size_t wcount = tiled ? d.n_freq * ceil_div(d.n_beams, 32) * d.n_ant * 32
+                                    : d.n_beams * d.n_freq * d.n_ant;
+        if (weights == nullptr || weight_count != wcount)
+            invalid("weight count does not match the kernel's weight layout");
+        bf_dims od = d;
+        if (r->spectra != 0) od.n_time = integrated_time_count(d.n_time, r->spectra);
+        const size_t n_out = od.n_time * od.n_freq * od.n_beams;
+        const bool int8 = r->output == BF_OUTPUT_INT8;
+        const size_t n_par = int8 ? quantization_parameter_count(od) : 0;
+        if (int8) {
+            if (codes == nullptr || code_count < n_out)
+                invalid("code buffer is smaller than the frame's integrated output");
+            if (parameters == nullptr || parameter_count < n_par)
+                invalid("parameter buffer is smaller than the frame's chunk count");
+        } else if (float_output == nullptr || float_count < n_out) {
+            invalid("float output buffer is smaller than the frame's output");
+        }
+        bf_timings t{};
+        rethrow_status(bf_workspace_setup_ms(r->ws, &t.setup_ms));
+        DeviceBuffer<uint8_t> dx(packed_bytes);
+        DeviceBuffer<float> dw(2 * wcount);
+        const auto h0 = Clock::now();
+        hip_check(hipMemcpy(dx.get(), packed, packed_bytes, hipMemcpyHostToDevice),
+                  "hipMemcpy offline voltage host to device");
+        hip_check(hipMemcpy(dw.get(), weights, 2 * wcount * sizeof(float),
+                            hipMemcpyHostToDevice),
+                  "hipMemcpy offline weights host to device");
+        t.host_to_device_ms =
+            std::chrono::duration<double, std::milli>(Clock::now() - h0).count();
+        const bf_frame_view vin =
+            contiguous_view(dx.get(), BF_DTYPE_PACKED_INT4X2, 3,
+                            {d.n_time, d.n_freq, d.n_ant, 0}, frame_id, *shard);
+        const bf_frame_view vw =
+            tiled ? contiguous_view(dw.get(), BF_DTYPE_COMPLEX_FLOAT32, 4,
+                                    {d.n_freq, ceil_div(d.n_beams, 32), d.n_ant, 32}, frame_id,
+                                    *shard)
+                  : contiguous_view(dw.get(), BF_DTYPE_COMPLEX_FLOAT32, 3,
+                                    {d.n_beams, d.n_freq, d.n_ant, 0}, frame_id, *shard);
+        DeviceBuffer<float> dout_f;
+        DeviceBuffer<int8_t> dout_q;
+        DeviceBuffer<float> dpar;
+        bf_frame_view vout{}, vpar{};
+        if (int8) {
+            dout_q.allocate(n_out);
+            dpar.allocate(2 * n_par);
+            vout = contiguous_view(dout_q.get(), BF_DTYPE_SIGNED_INT8, 3,
+                                   {od.n_time, od.n_freq, od.n_beams, 0}, frame_id, *shard);
+            vpar = contiguous_view(dpar.get(), BF_DTYPE_FLOAT32, 4,
+                                   {od.n_time, ceil_div(od.n_freq, 16), ceil_div(od.n_beams, 16),
+                                    2},
+                                   frame_id, *shard);
+        } else {
+            dout_f.allocate(n_out);
+            vout = contiguous_view(dout_f.get(), BF_DTYPE_FLOAT32, 3,
+                                   {od.n_time, od.n_freq, od.n_beams, 0}, frame_id, *shard);
+        }
+        bf_timings dt{};
+        rethrow_status(bf_workspace_run_device_frame(r->ws, &vin, &vw, &vout,
+                                                     int8 ? &vpar : nullptr, &dt));
+        t.kernel_ms = dt.kernel_ms;
+        t.temporal_integration_ms = dt.temporal_integration_ms;
+        t.quantization_ms = dt.quantization_ms;
+        t.device_to_device_ms = dt.device_to_device_ms;
+        const auto d0 = Clock::now();
+        if (int8) {
+            hip_check(hipMemcpy(codes, dout_q.get(), n_out, hipMemcpyDeviceToHost),
+                      "hipMemcpy offline int8 output device to host");
+            hip_check(hipMemcpy(parameters, dpar.get(), 2 * n_par * sizeof(float),
+                                hipMemcpyDeviceToHost),
+                      "hipMemcpy offline quantization parameters device to host");
+        } else {
+            hip_check(hipMemcpy(float_output, dout_f.get(), n_out * sizeof(float),
+                                hipMemcpyDeviceToHost),
+                      "hipMemcpy offline float output device to host");
+        }
+        t.device_to_host_ms =
+            std::chrono::duration<double, std::milli>(Clock::now() - d0).count();
+        if (output_dims) *output_dims = od;
+        if (timings) *timings = t;
     });
 }
 

@@ -287,6 +287,32 @@ bf_status bf_workspace_run_device_frame(bf_workspace* ws, const bf_frame_view* v
                                         bf_frame_view* quantization_parameters,
                                         bf_timings* timings);
 
+/* Reference: CudaOfflineFrameRunner (include/beamformer/cuda_offline_runner.hpp:
+ * 17-52, src/cuda_offline_runner.cu:130-228): one offline frame from host
+ * buffers.  Each run makes temporary device copies of the caller's voltages
+ * and weights, runs bf_workspace_run_device_frame on its own workspace and
+ * copies the result back: float output [T or W][F][B] (FLOAT32 runners) or
+ * int8 codes and {offset, scale} parameters (INT8 runners, which need
+ * integration).  Frame dims may shrink n_time and n_beams below the
+ * capacity, not change n_freq or n_ant; `weight_count` counts complex
+ * values in the kernel's layout (tiled for TILED).  `output_dims` (may be
+ * NULL) receives the output's dims.  Synchronous; not thread-safe. */
+typedef struct bf_offline_frame_runner bf_offline_frame_runner;
+bf_status bf_offline_frame_runner_create(const bf_dims* capacity, int kernel,
+                                         size_t integration_spectra, int output,
+                                         bf_offline_frame_runner** out);
+bf_status bf_offline_frame_runner_destroy(bf_offline_frame_runner* runner);
+bf_status bf_offline_frame_runner_config(const bf_offline_frame_runner* runner, int* kernel,
+                                         int* output, int* has_temporal_integration);
+bf_status bf_offline_frame_runner_run(bf_offline_frame_runner* runner, const uint8_t* packed,
+                                      size_t packed_bytes, const float* weights,
+                                      size_t weight_count, const bf_dims* dims,
+                                      uint64_t frame_id, const bf_shard* shard,
+                                      float* float_output, size_t float_count,
+                                      int8_t* codes, size_t code_count, float* parameters,
+                                      size_t parameter_count, bf_dims* output_dims,
+                                      bf_timings* timings);
+
 /* --------------------------------------------------- one-shot API */
 /* Reference: include/beamformer/cuda_beamformer.hpp:94-123 and
  * src/cuda_beamformer.cu:1081-1173: allocate-and-run on host buffers.

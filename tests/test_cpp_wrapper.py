@@ -24,3 +24,15 @@ def test_cpp_wrapper_maps_status_to_exceptions(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "OK" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_reference_named_cpp_api_compiles_and_links(tmp_path):
+    """include/beamformer/reference_api.hpp compiles with the reference's
+    names and links against the library (the GPU test runs it)."""
+    libdir = os.path.dirname(_capi.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror",
+                    "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "test_reference_api.cpp"),
+                    "-L", libdir, "-lbeamform_hip", f"-Wl,-rpath,{libdir}",
+                    "-o", str(tmp_path / "t")], check=True)
