@@ -94,6 +94,10 @@ struct bf_opt_tracker {
         int* mode = nullptr;
     } slot[2];
     hipStream_t prep = nullptr;
+    // ev_done: the end of the last run on its stream; the next run's stream
+    // (which may be another one) waits on it before it writes any state.
+    hipEvent_t ev_done = nullptr;
+    bool ran = false;
     hipEvent_t ev_start = nullptr, ev_ready[2] = {nullptr, nullptr},
                ev_used[2] = {nullptr, nullptr};
     float* d_part = nullptr;
@@ -112,7 +116,7 @@ struct bf_opt_tracker {
         for (void* p : {(void*)d_pos, (void*)d_freqs, (void*)d_state, (void*)d_part,
                         (void*)d_search, (void*)d_dirs, (void*)d_traj})
             if (p) (void)hipFree(p);
-        for (hipEvent_t e : {ev_start, ev_ready[0], ev_ready[1], ev_used[0], ev_used[1]})
+        for (hipEvent_t e : {ev_start, ev_ready[0], ev_ready[1], ev_used[0], ev_used[1], ev_done})
             if (e) (void)hipEventDestroy(e);
         if (prep) (void)hipStreamDestroy(prep);
     }
@@ -148,6 +152,11 @@ struct bf_opt_tracker {
         std::vector<float> prior(3 * W);
         for (size_t w = 0; w < (G > 1 ? 1 : W); ++w) window_direction(start, rate, w, S, &prior[3 * w]);
         ensure_dirs(W);
+        // Everything below (search state, trajectory, slot operands, the
+        // direction table, the prep stream's first windows) is ordered after
+        // the previous run, whichever stream it ran on: prep starts from
+        // ev_start, recorded on `stream` after this wait.
+        if (ran) hip_check(hipStreamWaitEvent(stream, ev_done, 0), "hipStreamWaitEvent");
         TrackerLaunch L_emit;
         L_emit.x = x;
         L_emit.out = out;
@@ -170,6 +179,8 @@ struct bf_opt_tracker {
             L_emit.dir_source = kDirTrajectory;
             L_emit.trajectories = d_traj;
             launch_tracker(stream, L_emit);
+            hip_check(hipEventRecord(ev_done, stream), "hipEventRecord");
+            ran = true;
             h_dirs = prior;
             dirs_pending = false;
             n_windows = W;
@@ -212,6 +223,8 @@ struct bf_opt_tracker {
         L_emit.dir_source = kDirTable;
         L_emit.dir_table = d_dirs;
         launch_tracker(stream, L_emit);
+        hip_check(hipEventRecord(ev_done, stream), "hipEventRecord");
+        ran = true;
         n_windows = W;
         dirs_pending = true;
     }
@@ -308,7 +321,7 @@ void create_tracker(const float* positions, const float* frequencies, const bf_d
     }
     // default priority (a stream of another priority is never created here)
     hip_check(hipStreamCreateWithFlags(&t->prep, hipStreamNonBlocking), "hipStreamCreate");
-    for (hipEvent_t* e : {&t->ev_start, &t->ev_ready[0], &t->ev_ready[1], &t->ev_used[0],
+    for (hipEvent_t* e : {&t->ev_done, &t->ev_start, &t->ev_ready[0], &t->ev_ready[1], &t->ev_used[0],
                           &t->ev_used[1]})
         hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
     t->d_part = dmalloc<float>(F * std::max<size_t>(t->G * t->G, 9));

@@ -97,6 +97,39 @@ def test_device_run_and_stateless_free_function():
     tracker_close(got, c["intensity"][0], 1e-4, 1e-5)
 
 
+def test_runs_on_two_streams_stay_ordered():
+    """Consecutive device runs issued on two different streams with no host
+    synchronisation in between: the second run waits for the first one
+    (which still reads the state it would overwrite), so both outputs and
+    the final directions equal the same two runs made one after another."""
+    c = golden("opt_tracker_capon_stateful")
+    x = case_input(c)
+    n, T, A = x.shape[0], x.shape[1], x.shape[3]
+    dims = api.Dimensions(T, F, A, 1)
+    start = tuple(float(v) for v in c["start"])
+    dx = [api.DeviceArray.from_numpy(x[k]) for k in range(2)]
+
+    def two_runs(streams):
+        t = OT.OptBeamTracker(None, None, dims, config(c))
+        t.seed_trajectory(start)
+        outs = [api.DeviceArray.empty((T, F, 1), np.float32) for _ in range(2)]
+        for k in range(2):
+            t.run_device(dx[k], outs[k], streams[k])
+            if streams[0] is streams[1]:
+                streams[k].synchronize()
+        for s_ in streams:
+            s_.synchronize()
+        d = t.window_directions()
+        t.close()
+        return [o.to_numpy() for o in outs], d
+
+    s1 = api.Stream()
+    ref, ref_dirs = two_runs([s1, s1])
+    got, got_dirs = two_runs([api.Stream(), api.Stream()])
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert np.array_equal(got_dirs, ref_dirs)
+
+
 def test_scan_disabled_equals_open_loop_tracker():
     c = golden("opt_tracker_scan_off")
     x = case_input(c)[0]

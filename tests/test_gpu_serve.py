@@ -122,6 +122,21 @@ def test_alone_equals_mixed(stream, mixed):
         assert np.array_equal(alone[0][1], out[i][1]), f"request {i}"
 
 
+def test_alone_equals_mixed_on_the_full_size_ring(stream, mixed):
+    """The same request mix on a 2^20-spectrum ring (22.5 GB at A=64,
+    F=336) with a 1 GiB output budget: each request alone equals the mix,
+    and both equal the small-ring results (the ring size changes only where
+    a spectrum lives)."""
+    big = dict(ring_spectra=1 << 20, hbm_budget_bytes=1 << 30, max_step_spectra=8192)
+    out_big, _ = run(MIX, stream, **big)
+    out, _ = mixed
+    for i, req in enumerate(MIX[:3]):
+        assert out_big[i][0] == out[i][0]
+        assert np.array_equal(out_big[i][1], out[i][1]), i
+        alone, _ = run([(req[0], req[1], req[2], out_big[i][0], req[4], req[5])], stream, **big)
+        assert np.array_equal(alone[0][1], out_big[i][1]), i
+
+
 def test_matches_multibeam_tracker_and_fp64(stream, mixed):
     out, stats = mixed
     for i, (B, S, n, _, _, seed) in enumerate(MIX):
