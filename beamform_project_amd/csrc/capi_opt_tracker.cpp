@@ -270,7 +270,8 @@ void create_tracker(const float* positions, const float* frequencies, const bf_d
     if (kernel != BF_KERNEL_FP32 && kernel != BF_KERNEL_SPLIT_F16 && kernel != BF_KERNEL_F16 &&
         kernel != BF_KERNEL_AUTO)
         invalid("tracker kernel must be FP32, SPLIT_F16, F16 or AUTO");
-    if (kernel == BF_KERNEL_AUTO) kernel = BF_KERNEL_SPLIT_F16;
+    // AUTO stays AUTO: the one-beam emission then takes the same kernel as
+    // the open-loop tracker's AUTO (the FP32 stream kernel).
     auto t = std::make_unique<bf_opt_tracker>();
     t->dims = d;
     t->cfg = *c;

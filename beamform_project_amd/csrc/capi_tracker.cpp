@@ -27,7 +27,9 @@ int tracker_kernel(const bf_tracker_exec* exec) {
     if (k != BF_KERNEL_FP32 && k != BF_KERNEL_SPLIT_F16 && k != BF_KERNEL_F16 &&
         k != BF_KERNEL_AUTO)
         invalid("tracker kernel must be FP32, SPLIT_F16, F16 or AUTO");
-    return k == BF_KERNEL_AUTO ? BF_KERNEL_SPLIT_F16 : k;
+    // AUTO stays AUTO: one beam with per-spectrum output takes the FP32
+    // stream kernel, everything else the split-FP16 MFMA kernel.
+    return k;
 }
 
 // Reference src/geometry.cpp:97-106 (direction_from_lm), float arithmetic.

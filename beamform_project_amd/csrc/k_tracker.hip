@@ -31,6 +31,8 @@
 // of 16 consecutive spectra (rows past the window are zero), so one B
 // operand serves a whole tile.  Integrated output folds the four row groups
 // of a lane column with two cross-lane adds at the window's last tile.
+#include <algorithm>
+
 #include "bf_kernels.hpp"
 #include "bf_serve.hpp"
 #include "mfma_stage.hpp"
@@ -1022,9 +1024,130 @@ void launch_fp32(hipStream_t stream, const TrackerLaunch& L, const DirArgs& da) 
 #undef BF_FP32
 }
 
+// Single-beam per-spectrum tracker as an HBM stream (FP32 VALU).  Every
+// lane holds 32 antennas of one channel, LPC = A / 32 lanes per channel,
+// so a wave covers 64 / LPC adjacent channels and reads, per spectrum, one
+// contiguous 2 KB run of packed rows (two 16-byte loads per lane, the next
+// spectrum's issued before the current one is summed).  A unit is (window,
+// channel group, time chunk); its weights (reduced-phase FP64 -> FP32
+// sincos, within about one float ulp of (float)cos / (float)sin of the FP64
+// phase) stay in 64 registers for the whole chunk.  The complex sum runs
+// over the lane's antennas in order in two interleaved accumulator pairs,
+// then across the LPC lanes of the channel by xor shuffles.
+constexpr int kB1WavesPerBlock = 4;
+
+template <int A, int DIRSRC>
+__global__ __launch_bounds__(kB1WavesPerBlock * 64) void beam1_stream_kernel(
+    const uint8_t* __restrict__ x, float* __restrict__ out, int T, int F, int S, int n_windows,
+    int groups, int chunks, Geom geo, DirArgs da) {
+#pragma clang fp contract(off)
+    constexpr int LPC = A / 32;
+    constexpr int CPW = 64 / LPC;  // channels per wave
+    const int lane = threadIdx.x & 63;
+    const long unit = (long)blockIdx.x * kB1WavesPerBlock + (threadIdx.x >> 6);
+    if (unit >= (long)n_windows * groups * chunks) return;  // whole waves
+    const int c = (int)(unit % chunks);
+    const long r = unit / chunks;
+    const int g = (int)(r % groups);
+    const int w = (int)(r / groups);
+    const int f = g * CPW + lane / LPC;
+    const int part = lane % LPC;  // antennas 32 part .. 32 part + 31
+    const bool live = f < F;
+    const int fl = live ? f : F - 1;  // loads stay in bounds; nothing is stored
+    const int tw0 = w * S;
+    const int tw1 = min(tw0 + S, T);
+    const int per = (tw1 - tw0 + chunks - 1) / chunks;
+    const int t0 = min(tw0 + c * per, tw1);
+    const int t1 = min(t0 + per, tw1);
+    float d[3];
+    beam_direction<DIRSRC>(da, w, 0, d);
+    const double k = wave_number(channel_frequency(geo, fl));
+    float wr[32], wi[32];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+        float p[3];
+        antenna_position(geo, 32 * part + j, p);
+        const float2 v = steering_weight_reduced(p, d, k);
+        wr[j] = v.x;
+        wi[j] = v.y;
+    }
+    const uint8_t* base = x + (size_t)fl * A + 32 * part;
+    const size_t row = (size_t)F * A;
+    uint4 cur[2], nxt[2];
+    if (t0 < t1) {
+        const uint4* q = reinterpret_cast<const uint4*>(base + (size_t)t0 * row);
+        cur[0] = q[0];
+        cur[1] = q[1];
+    }
+    for (int t = t0; t < t1; ++t) {
+        if (t + 1 < t1) {
+            const uint4* q = reinterpret_cast<const uint4*>(base + (size_t)(t + 1) * row);
+            nxt[0] = q[0];
+            nxt[1] = q[1];
+        }
+        float re[2] = {0.0f, 0.0f}, im[2] = {0.0f, 0.0f};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const unsigned wd[4] = {cur[h].x, cur[h].y, cur[h].z, cur[h].w};
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int j = 16 * h + q;
+                const int byte = (int)((wd[q >> 2] >> (8 * (q & 3))) & 0xFFu);
+                const float xr = (float)((byte << 28) >> 28);
+                const float xi = (float)((byte << 24) >> 28);
+                const int e = q & 1;
+                re[e] = fmaf(wr[j], xr, re[e]);
+                re[e] = fmaf(-wi[j], xi, re[e]);
+                im[e] = fmaf(wr[j], xi, im[e]);
+                im[e] = fmaf(wi[j], xr, im[e]);
+            }
+        }
+        float sr = re[0] + re[1], si = im[0] + im[1];
+#pragma unroll
+        for (int o = 1; o < LPC; o <<= 1) {
+            sr += __shfl_xor(sr, o);
+            si += __shfl_xor(si, o);
+        }
+        if (live && part == 0) out[(size_t)t * F + f] = sr * sr + si * si;
+        cur[0] = nxt[0];
+        cur[1] = nxt[1];
+    }
+}
+
+// Units (window, channel group, time chunk) for about four waves per SIMD,
+// chunks of at least 8 spectra.
+template <int DIRSRC>
+void launch_beam1_stream(hipStream_t stream, const TrackerLaunch& L, const DirArgs& da) {
+    const size_t cpw = 64 / (L.A / 32);
+    const size_t groups = ceil_div(L.F, cpw);
+    const size_t target = 4096;
+    size_t chunks = std::max<size_t>(1, ceil_div(target, L.n_windows * groups));
+    chunks = std::min(chunks, std::max<size_t>(1, L.S / 8));
+    const size_t n_units = L.n_windows * groups * chunks;
+    const size_t blocks = ceil_div(n_units, kB1WavesPerBlock);
+    if (blocks > (size_t)UINT32_MAX) fail(BF_ERR_OVERFLOW, "tracker grid exceeds the supported size");
+#define BF_B1(A_)                                                                              \
+    hipLaunchKernelGGL((beam1_stream_kernel<A_, DIRSRC>), dim3((unsigned)blocks),               \
+                       dim3(kB1WavesPerBlock * 64), 0, stream, L.x, L.out, (int)L.T, (int)L.F, \
+                       (int)L.S, (int)L.n_windows, (int)groups, (int)chunks, L.geo, da)
+    switch (L.A) {
+    case 32: BF_B1(32); break;
+    case 64: BF_B1(64); break;
+    case 128: BF_B1(128); break;
+    case 256: BF_B1(256); break;
+    default: invalid("n_ant must be 32, 64, 128, or 256");
+    }
+#undef BF_B1
+}
+
 template <int DIRSRC>
 void dispatch(hipStream_t stream, const TrackerLaunch& L, const DirArgs& da) {
     const bool integ = L.integrate;
+    // One beam, per-spectrum output: the FP32 stream kernel (AUTO and FP32).
+    if (L.B == 1 && !integ && (L.kernel == BF_KERNEL_AUTO || L.kernel == BF_KERNEL_FP32)) {
+        launch_beam1_stream<DIRSRC>(stream, L, da);
+        return;
+    }
     switch (L.kernel) {
     case BF_KERNEL_SPLIT_F16:
     case BF_KERNEL_AUTO:

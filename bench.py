@@ -463,7 +463,7 @@ def bench_b1(args):
     dims = api.Dimensions(T, F_, A, 1)
     cfg = TR.TrackerConfig(TR.direction_from_lm(0.0, 0.0), (1e-5, 0.0), S)
     res = {}
-    for name in ("SPLIT_F16", "FP32"):
+    for name in ("AUTO", "SPLIT_F16", "FP32"):
         ex = TR.TrackerExec(api.Kernel[name])
 
         def launch():
@@ -475,12 +475,18 @@ def bench_b1(args):
     ms = res[best]["ms_per_step"]
     value = T * F_ / (ms * 1e-3)
     return common_line(
-        args, "beam-samples/s (T·F·B)", value, "beam-samples/s", ms, "f32" if best == "FP32"
-        else "f16 split hi+lo, f32 accumulate",
+        args, "beam-samples/s (T·F·B)", value, "beam-samples/s", ms,
+        "f16 split hi+lo, f32 accumulate" if best == "SPLIT_F16" else "f32",
         {"workload": "B=1 dynamic tracker, device-resident: A=64 (reference grid 0.6 m), "
                      "F=336, T=15360, S=320, per-spectrum output, uniform int4",
          "n_ant": A, "n_freq": F_, "n_time": T, "n_beams": 1, "integration_spectra": S,
          "kernel": best},
+        roofline={"bound": "hbm",
+                  "achieved": round((T * F_ * A + T * F_ * 4) / (res[best]["kernel_avg_ms"] * 1e-3) / 1e9, 1),
+                  "peak": 8000.0, "unit": "GB/s",
+                  "frac": round((T * F_ * A + T * F_ * 4) / (res[best]["kernel_avg_ms"] * 1e-3) / 8e12, 4),
+                  "kernel_avg_ms": res[best]["kernel_avg_ms"],
+                  "algorithmic_bytes": T * F_ * A + T * F_ * 4},
         vs_baseline=round(value / 3.59e8, 2),
         baseline_note="reference CUDA V5 device-resident A=64: 14.38 ms = 3.59e8 "
                       "beam-samples/s on a Quadro P1000",
