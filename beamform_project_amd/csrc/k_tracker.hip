@@ -1027,14 +1027,15 @@ void launch_fp32(hipStream_t stream, const TrackerLaunch& L, const DirArgs& da) 
 // Single-beam per-spectrum tracker as an HBM stream (FP32 VALU).  Every
 // lane holds 32 antennas of one channel, LPC = A / 32 lanes per channel,
 // so a wave covers 64 / LPC adjacent channels and reads, per spectrum, one
-// contiguous 2 KB run of packed rows (two 16-byte loads per lane, the next
-// spectrum's issued before the current one is summed).  A unit is (window,
+// contiguous 2 KB run of packed rows (two 16-byte loads per lane, with the
+// next kB1Depth - 1 spectra in flight while one is summed).  A unit is (window,
 // channel group, time chunk); its weights (reduced-phase FP64 -> FP32
 // sincos, within about one float ulp of (float)cos / (float)sin of the FP64
 // phase) stay in 64 registers for the whole chunk.  The complex sum runs
 // over the lane's antennas in order in two interleaved accumulator pairs,
 // then across the LPC lanes of the channel by xor shuffles.
 constexpr int kB1WavesPerBlock = 4;
+constexpr int kB1Depth = 4;  // spectra in flight per lane
 
 template <int A, int DIRSRC>
 __global__ __launch_bounds__(kB1WavesPerBlock * 64) void beam1_stream_kernel(
@@ -1073,44 +1074,51 @@ __global__ __launch_bounds__(kB1WavesPerBlock * 64) void beam1_stream_kernel(
     }
     const uint8_t* base = x + (size_t)fl * A + 32 * part;
     const size_t row = (size_t)F * A;
-    uint4 cur[2], nxt[2];
-    if (t0 < t1) {
-        const uint4* q = reinterpret_cast<const uint4*>(base + (size_t)t0 * row);
-        cur[0] = q[0];
-        cur[1] = q[1];
-    }
-    for (int t = t0; t < t1; ++t) {
-        if (t + 1 < t1) {
-            const uint4* q = reinterpret_cast<const uint4*>(base + (size_t)(t + 1) * row);
-            nxt[0] = q[0];
-            nxt[1] = q[1];
+    // a ring of kB1Depth spectra in flight: slot u holds spectrum tb + u and
+    // is refilled with spectrum tb + u + kB1Depth once it has been summed
+    uint4 ring[kB1Depth][2];
+#pragma unroll
+    for (int u = 0; u < kB1Depth; ++u)
+        if (t0 + u < t1) {
+            const uint4* q = reinterpret_cast<const uint4*>(base + (size_t)(t0 + u) * row);
+            ring[u][0] = q[0];
+            ring[u][1] = q[1];
         }
-        float re[2] = {0.0f, 0.0f}, im[2] = {0.0f, 0.0f};
+    for (int tb = t0; tb < t1; tb += kB1Depth) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const unsigned wd[4] = {cur[h].x, cur[h].y, cur[h].z, cur[h].w};
+        for (int u = 0; u < kB1Depth; ++u) {
+            const int t = tb + u;
+            if (t >= t1) break;  // uniform
+            float re[2] = {0.0f, 0.0f}, im[2] = {0.0f, 0.0f};
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int j = 16 * h + q;
-                const int byte = (int)((wd[q >> 2] >> (8 * (q & 3))) & 0xFFu);
-                const float xr = (float)((byte << 28) >> 28);
-                const float xi = (float)((byte << 24) >> 28);
-                const int e = q & 1;
-                re[e] = fmaf(wr[j], xr, re[e]);
-                re[e] = fmaf(-wi[j], xi, re[e]);
-                im[e] = fmaf(wr[j], xi, im[e]);
-                im[e] = fmaf(wi[j], xr, im[e]);
+            for (int h = 0; h < 2; ++h) {
+                const unsigned wd[4] = {ring[u][h].x, ring[u][h].y, ring[u][h].z, ring[u][h].w};
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int j = 16 * h + q;
+                    const int byte = (int)((wd[q >> 2] >> (8 * (q & 3))) & 0xFFu);
+                    const float xr = (float)((byte << 28) >> 28);
+                    const float xi = (float)((byte << 24) >> 28);
+                    const int e = q & 1;
+                    re[e] = fmaf(wr[j], xr, re[e]);
+                    re[e] = fmaf(-wi[j], xi, re[e]);
+                    im[e] = fmaf(wr[j], xi, im[e]);
+                    im[e] = fmaf(wi[j], xr, im[e]);
+                }
             }
-        }
-        float sr = re[0] + re[1], si = im[0] + im[1];
+            if (t + kB1Depth < t1) {
+                const uint4* q = reinterpret_cast<const uint4*>(base + (size_t)(t + kB1Depth) * row);
+                ring[u][0] = q[0];
+                ring[u][1] = q[1];
+            }
+            float sr = re[0] + re[1], si = im[0] + im[1];
 #pragma unroll
-        for (int o = 1; o < LPC; o <<= 1) {
-            sr += __shfl_xor(sr, o);
-            si += __shfl_xor(si, o);
+            for (int o = 1; o < LPC; o <<= 1) {
+                sr += __shfl_xor(sr, o);
+                si += __shfl_xor(si, o);
+            }
+            if (live && part == 0) out[(size_t)t * F + f] = sr * sr + si * si;
         }
-        if (live && part == 0) out[(size_t)t * F + f] = sr * sr + si * si;
-        cur[0] = nxt[0];
-        cur[1] = nxt[1];
     }
 }
 

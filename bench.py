@@ -16,6 +16,8 @@ workloads, each with its own value, roofline and CPU baseline:
   c1  plain fixed grid (A=32 ULA, T=1024, B=16, f32 per-spectrum output),
   c3  dynamic tracker over the full band (F=672, T=65536, S=320, B=32),
   c5  adaptive DOA update (K=1024 snapshots x 336 channels, M=48, 721 angles),
+  b1  the reference's published single-beam tracker case (A=64, T=15360,
+      S=320, per-spectrum output), the one figure the reference publishes,
 and under `serving` the streaming server under Poisson request arrivals at
 the stream's real-time rate (C4).  `--workload X` prints one of them alone.
 
@@ -243,7 +245,7 @@ def main():
 
     legs = {}
     if not args.no_legs:
-        for name, fn in (("c1", bench_c1), ("c3", bench_c3), ("c5", bench_c5)):
+        for name, fn in (("c1", bench_c1), ("c3", bench_c3), ("c5", bench_c5), ("b1", bench_b1)):
             legs[name] = fn(args)
 
     line = {
