@@ -188,6 +188,7 @@ struct bf_server {
     std::deque<uint64_t> pending;
     std::vector<uint64_t> slot_owner;  // 0: free
     std::vector<int> dirty_slots;
+    size_t active_beams = 0;  // beams of the admitted, unfinished requests
     long long head = 0;       // spectra ingested
     long long processed = 0;  // head released to the last issued step
     long long done_head = 0;  // head of the last harvested step
@@ -304,6 +305,7 @@ struct bf_server {
             r.windows_done = std::max(r.windows_done, sr.w_hi);
             if (r.state == kActive && r.windows_done == r.windows_total) {
                 r.state = kComplete;
+                active_beams -= r.B;
                 ++n_completed;
             }
         }
@@ -362,6 +364,8 @@ struct bf_server {
                     break;
                 }
             if (s < 0) return;
+            if (cfg.max_active_beams != 0 && active_beams + r.B > cfg.max_active_beams)
+                return;  // first in, first admitted
             size_t off = 0;
             if (!arena.alloc(r.arena_bytes, &off)) return;  // first in, first admitted
             r.arena_off = off;
@@ -370,6 +374,7 @@ struct bf_server {
             r.t_end = r.t0 + (long long)r.n_spectra;
             r.admit_head = head;
             r.state = kActive;
+            active_beams += r.B;
             slot_owner[s] = r.id;
             ServeSlot& d = h_staging[s];
             std::memset(&d, 0, sizeof(d));
@@ -571,6 +576,7 @@ struct bf_server {
                 // Cancel: no step may still write into its outputs.
                 hip_check(hipStreamSynchronize(step_stream), "hipStreamSynchronize");
                 harvest(true);
+                if (r.state == kActive) active_beams -= r.B;
             }
             arena.release(r.arena_off, r.arena_bytes);
             h_staging[r.slot].active = 0;
@@ -744,6 +750,8 @@ void submit_request(bf_server* s, const bf_server_request* q, uint64_t* id) {
     require_ptr(q->trajectories, "trajectories");
     if (q->n_beams == 0 || q->n_beams > (size_t)kServeMaxBeams)
         invalid("n_beams must be in 1.." + std::to_string(kServeMaxBeams));
+    if (s->cfg.max_active_beams != 0 && q->n_beams > s->cfg.max_active_beams)
+        invalid("n_beams exceeds the server's max_active_beams");
     if (q->window == 0 || q->window > s->cfg.max_window)
         invalid("window must be in 1..max_window (" + std::to_string(s->cfg.max_window) + ")");
     if (q->n_spectra == 0) invalid("n_spectra must be positive");

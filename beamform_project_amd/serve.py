@@ -24,7 +24,8 @@ class ServerConfigC(C.Structure):
     _fields_ = [("n_freq", _sz), ("n_ant", _sz), ("ring_spectra", _sz), ("max_window", _sz),
                 ("max_step_spectra", _sz), ("hbm_budget_bytes", _sz), ("max_requests", _sz),
                 ("positions", _vp), ("frequencies", _vp), ("kernel", C.c_int),
-                ("enable_graph", C.c_int), ("loss_masks", C.c_int), ("shard", _vp)]
+                ("enable_graph", C.c_int), ("loss_masks", C.c_int), ("shard", _vp),
+                ("max_active_beams", _sz)]
 
 
 class RingSpanC(C.Structure):
@@ -118,6 +119,7 @@ class ServerConfig:
     enable_graph: bool = True
     loss_masks: bool = False                   # validity ring [ring][F]
     shard: Optional[object] = None             # formats.ShardDescriptor of the stream
+    max_active_beams: int = 0                  # compute budget for admission; 0: none
 
     def to_c(self, keep: list) -> ServerConfigC:
         pos = fr = None
@@ -136,7 +138,8 @@ class ServerConfig:
                              None if pos is None else pos.ctypes.data,
                              None if fr is None else fr.ctypes.data, int(self.kernel),
                              int(self.enable_graph), int(self.loss_masks),
-                             None if sh is None else C.addressof(sh))
+                             None if sh is None else C.addressof(sh),
+                             int(self.max_active_beams))
 
 
 @dataclass

@@ -236,7 +236,7 @@ def main():
     serving = sweep = None
     if not args.no_serving:
         serving = serving_run(args.serve_seconds)
-        sweep = serving_sweep(3.0)
+        sweep = serving_sweep(3.0) + serving_sweep(3.0, (12.0, 24.0), max_active_beams=320)
 
     cpu = cpu_all = None
     if not args.no_cpu_baseline:
@@ -706,14 +706,17 @@ def fp64_error(x, out, w, A, B, S, n_windows=32):
     return float(np.max(np.abs(got - p) / np.maximum(p, floor)))
 
 
-def serving_sweep(seconds, rates=(6.0, 12.0, 24.0, 48.0)):
+def serving_sweep(seconds, rates=(6.0, 12.0, 24.0, 48.0), max_active_beams=0):
     """The serving loop at rising offered loads (requests/s), `seconds` of
     stream each, until p99 passes the 0.5 ms budget (that load is the last
-    one reported)."""
+    one reported); with max_active_beams, admission also caps the beams in
+    flight (requests then wait, reported as the mean admission wait)."""
     rows = []
     for r in rates:
-        sv = serving_run(seconds, r)
-        rows.append({"arrival_rate_hz": r, "latency_p50_ms": sv["latency_p50_ms"],
+        sv = serving_run(seconds, r, max_active_beams)
+        rows.append({"arrival_rate_hz": r, "max_active_beams": max_active_beams,
+                     "mean_admission_wait_ms": sv["mean_admission_wait_ms"],
+                     "latency_p50_ms": sv["latency_p50_ms"],
                      "latency_p99_ms": sv["latency_p99_ms"],
                      "latency_max_ms": sv["latency_max_ms"],
                      "gpu_busy_frac": sv["gpu_busy_frac"],
@@ -725,7 +728,7 @@ def serving_sweep(seconds, rates=(6.0, 12.0, 24.0, 48.0)):
     return rows
 
 
-def serving_run(seconds, rate_hz=6.0):
+def serving_run(seconds, rate_hz=6.0, max_active_beams=0):
     """The streaming server under Poisson arrivals, paced at 2.56 us per
     spectrum (the native load generator bf_server_simulate)."""
     from beamform_project_amd import serve as SV
@@ -737,7 +740,7 @@ def serving_run(seconds, rate_hz=6.0):
         server=SV.ServerConfig(n_freq=F, n_ant=64, ring_spectra=1 << 20, max_window=320,
                                max_step_spectra=8192, hbm_budget_bytes=2 << 30,
                                max_requests=256, positions=pos, frequencies=fr,
-                               enable_graph=True),
+                               enable_graph=True, max_active_beams=max_active_beams),
         seed=4242, cadence_s=2.56e-6, run_seconds=seconds, chunk_spectra=128,
         arrival_rate_hz=rate_hz, min_beams=1, max_beams=64, min_duration_s=0.5,
         max_duration_s=5.0, window_a=10, window_b=320, output_windows_cap=1024, paced=True,

@@ -657,6 +657,9 @@ typedef struct bf_server_config {
     int loss_masks;           /* 1: keep a validity ring and honour loss masks */
     const bf_shard* shard;    /* the stream's shard (copied); with frequencies NULL its
                                  channels of the reference plan; NULL: shard 0 */
+    size_t max_active_beams;  /* compute budget: a request is admitted only while the
+                                 beams of the active requests plus its own stay within
+                                 it (FIFO, like the HBM budget); 0: no cap */
 } bf_server_config;
 
 /* Ring rows of consecutive spectra handed to a producer: voltages

@@ -228,6 +228,44 @@ def test_admission_waits_for_hbm_budget(stream):
     srv2.close()
 
 
+def test_admission_waits_for_the_beam_budget(stream):
+    """With max_active_beams, a request waits (FIFO) until the active
+    requests' beams plus its own fit; it is admitted at completion of an
+    earlier one (no release needed, the budget is compute, not memory), and
+    its output equals a run on a server without the cap."""
+    S, n = 10, 300
+    srv = SV.TrackerServer(config(max_active_beams=40))
+    with pytest.raises(_capi.InvalidArgument, match="max_active_beams"):
+        srv.submit(SV.Request(trajectories(41, 1), S, n))
+    ids = [srv.submit(SV.Request(trajectories(B, 50 + i), S, n))
+           for i, B in enumerate((24, 16, 9))]
+    srv.step()
+    assert [srv.status(r)["state"] for r in ids] == [SV.ACTIVE, SV.ACTIVE, SV.PENDING]
+    h = 0
+    while srv.status(ids[2])["state"] == SV.PENDING:
+        srv.ingest(stream[h:h + 100])
+        h += 100
+        srv.step()
+        srv.synchronize()
+    st2 = srv.status(ids[2])
+    assert srv.status(ids[0])["state"] == SV.COMPLETE and st2["start"] > 0
+    while srv.status(ids[2])["state"] != SV.COMPLETE:
+        srv.ingest(stream[h:h + 100])
+        h += 100
+        srv.step()
+        srv.synchronize()
+    late = srv.read(ids[2], 0, n // S, 9)
+    srv.close()
+    srv2 = SV.TrackerServer(config())
+    r2 = srv2.submit(SV.Request(trajectories(9, 52), S, n, st2["start"]))
+    for g in range(0, h, 100):
+        srv2.ingest(stream[g:g + 100])
+        srv2.step()
+    srv2.synchronize()
+    assert np.array_equal(srv2.read(r2, 0, n // S, 9), late)
+    srv2.close()
+
+
 def test_output_ring_and_reads(stream):
     B, S, n = 3, 10, 500
     srv = SV.TrackerServer(config())
