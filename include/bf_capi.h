@@ -360,7 +360,10 @@ typedef struct bf_tracker_exec {
  * hard-coded geometry (default_positions at 0.6 m, channels at 300 MHz +
  * k * 300 kHz).  Validation (n_beams == 1, trajectory on the unit disk for
  * every window) happens before launch; synchronises only if stream is NULL.
- * Allocates nothing.  `exec` may be NULL (AUTO). */
+ * The 16-byte trajectory row travels in a stream-ordered allocation on
+ * `stream` (hipMallocAsync / hipFreeAsync); nothing else is allocated.
+ * `exec` may be NULL (AUTO: the FP32 stream kernel for this one-beam,
+ * per-spectrum case). */
 bf_status bf_beam_tracker_device_resident(const uint8_t* d_packed, float* d_intensity,
                                           const bf_dims* dims,
                                           const bf_tracker_config* tracker,
